@@ -1,0 +1,145 @@
+"""Benchmark of the SWM time-stepping hot path on one MI355X.
+
+Workload: the nonlinear beta-plane basin (1024 x 1024, dx = 5 km, f0 = 1e-4,
+beta = 2e-11, H = 500 m, Ah = 100 m^2/s, linear drag 1e-6 1/s, double-gyre
+windstress, ~5% seeded islands, one passive tracer, FP64, AB3, dt = 30 s).
+A step is one pass of SWM_timestep + TRC_timestep + model_advance over the
+whole grid, fields resident in HBM.
+
+value = cell-updates/s = Nx*Ny*K / (device time of the K timed steps).
+roofline.achieved = algorithmic bytes per step / average step-kernel time,
+where the algorithmic bytes per cell-step are
+  state u, v, eta read + write         6 x 8 = 48
+  AB history, 2 reads + 1 write x 3    9 x 8 = 72
+  packed ocean masks                           1
+  tracer CH read + write, G_CH 2r + 1w 5 x 8 = 40
+= 161 B (H, impl, windstress are row-uniform in this workload and travel as
+per-row tables).
+
+cpu_baseline: the reference-order C restatement (oracle/) on the host cores,
+timed on the same workload for a bounded number of steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s + % HBM roofline; served-ensemble p50/p99 latency vs plain"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = {"f64": 121 + 40, "f32": 61 + 20}
+
+
+def read_traffic(precision: str):
+    """HBM bytes per step-kernel launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", f"step_kernel_c2_1024_{precision}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch:"):
+                try:
+                    return float(line.split(":", 1)[1])
+                except ValueError:
+                    return None
+    return None
+
+
+def cpu_baseline(s, budget_s: float):
+    import oracle
+    o = oracle.Oracle(s)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o.step(1)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.step(1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 2000:
+            break
+    o.close()
+    return {"value": s.Nx * s.Ny * n / el, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"same {s.Nx}x{s.Ny} workload, {n} steps after 1 warm-up step ({el:.1f} s, "
+                      f"OpenMP schedule(static,5) over rows)"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=1024, help="grid size (default: the 1024 x 1024 workload)")
+    ap.add_argument("--dt", type=float, default=30.0)
+    ap.add_argument("--precision", choices=["f64", "f32"], default="f64")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every run here uses exactly one MI355X; --gpus must be 1",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 1
+    torch.cuda.set_device(0)
+    from swm_amd import configs, gpu
+
+    s = configs.c2_nonlinear(seed=0, N=args.n, dt=args.dt, nsteps=args.warmup + args.steps)
+    prec = gpu.FP32 if args.precision == "f32" else gpu.FP64
+    g = gpu.SwmGpu(s, precision=prec).init_from_setup()
+    g.step(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.step(args.steps)          # device events bracket the K launches on the library's stream
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms, step_ms = g.last_timing()
+    eta = g.get("SWM_ETA")
+    import numpy as np
+    finite = bool(np.isfinite(eta).all())
+    g.close()
+
+    cells = s.Nx * s.Ny
+    value = cells * args.steps / (total_ms * 1e-3)
+    bpc = BYTES_PER_CELL[args.precision]
+    achieved = bpc * cells / (step_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic",
+        "config": {
+            "workload": f"nonlinear beta-plane basin {s.Nx}x{s.Ny}, 1 tracer, 5% islands, AB3 dt={args.dt:g} s",
+            "Nx": s.Nx, "Ny": s.Ny, "dt": args.dt, "tracers": 1, "variant": "FULLY_NONLINEAR+LATERAL_MIXING",
+            "fields_finite_at_end": finite, "wall_s": wall,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(args.precision),
+            "bytes_per_cell_step": bpc,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(configs.c2_nonlinear(seed=0, N=args.n, dt=args.dt), args.cpu_seconds)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

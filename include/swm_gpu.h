@@ -1,0 +1,164 @@
+/*
+ * swm_gpu.h -- C-ABI of the MI355X (gfx950) SWM time-stepping path.
+ *
+ * The reference has no FFI layer: its "interface" is the set of Fortran module
+ * procedures that src/model.f90 calls in a fixed order, plus module arrays.
+ * Each entry point below replaces one group of them; a Fortran
+ * ISO_C_BINDING shim keeping the reference procedure names calls these
+ * (swm_amd/fortran/swm_gpu_shim.f90).
+ *
+ * Conventions
+ *  - Every array argument is a host pointer to a Fortran (Nx,Ny) array,
+ *    leading dimension Nx (C view [Ny][Nx]); (Nx,Ny,3) AB history arrays are
+ *    three such planes back to back.  Masks are int8 0/1 "ocean" masks.
+ *  - Host memory stays caller-owned and is only read/written inside the call.
+ *    Device memory is owned by the opaque context.
+ *  - Return value 0 = success; otherwise a non-zero status and
+ *    swm_gpu_last_error() gives the message.  No C++ exception crosses the ABI.
+ *  - One host thread per context; all device work is ordered on the context's
+ *    own stream; swm_gpu_sync/_sor/_get block until their results are on the host.
+ *  - The HIP path is the only path: there is no CPU fallback.  A missing or
+ *    unusable GPU makes swm_gpu_create fail.
+ */
+#ifndef SWM_GPU_H
+#define SWM_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWM_GPU_ABI_VERSION 1
+
+/* model.h physics variant (include/model.h:9-12 of each experiment) */
+enum swm_variant {
+    SWM_FULLY_NONLINEAR = 1,
+    SWM_LINEARISED_MEAN_STATE = 2,          /* not supported on the GPU path: create fails */
+    SWM_LINEARISED_STATE_OF_REST = 3
+};
+/* model.h switches */
+enum swm_flags {
+    SWM_LATERAL_MIXING = 1,                 /* LATERAL_MIXING */
+    SWM_QUADRATIC_BOTTOM_FRICTION = 2,      /* QUADRATIC_BOTTOM_FRICTION */
+    SWM_BAROTROPIC = 4,                     /* BAROTROPIC */
+    SWM_TAU_SCALE = 8                       /* TAU_SCALE defined (value in swm_desc.tau_scale) */
+};
+enum swm_precision { SWM_FP64 = 0, SWM_FP32 = 1 };
+enum swm_grid_id { SWM_GRID_H = 0, SWM_GRID_U = 1, SWM_GRID_V = 2, SWM_GRID_ETA = 3 };
+enum swm_component { SWM_COMP_X = 0, SWM_COMP_Y = 1, SWM_COMP_ETA = 2 };
+
+/* grid_t, src/grid_module.f90:9-20 (H, ocean, cos_lat, tan_lat, f) */
+typedef struct swm_grid_desc {
+    const int8_t *ocean;     /* (Nx,Ny) ocean mask; land = 1 - ocean */
+    const double *H;         /* (Nx,Ny) layer thickness on this grid as left by init */
+    const double *cos_lat;   /* (Ny) */
+    const double *tan_lat;   /* (Ny) */
+    const double *f;         /* (Ny) Coriolis parameter */
+} swm_grid_desc;
+
+/* Everything the step reads that the reference computes at init:
+ * initDomain (src/domain_module.f90:49-207), SWM_damping_init
+ * (src/swm_damping_module.f90:44-117), SWM_forcing_init
+ * (src/swm_forcing_module.f90:78-127), time_integration_init
+ * (src/time_integration_module.f90:35-52), vars_module constants
+ * (src/vars_module.f90:27-40).  NULL array pointers take the reference's
+ * initVar defaults (impl = 1, gamma = 0, forcing = 0). */
+typedef struct swm_desc {
+    int32_t abi_version;     /* SWM_GPU_ABI_VERSION */
+    int32_t Nx, Ny;
+    int32_t variant;         /* enum swm_variant */
+    int32_t flags;           /* enum swm_flags */
+    int32_t precision;       /* enum swm_precision */
+    double G, Ah, dt, A, dLambda, dTheta, RHO0, minD, lbc, ab_chi, tau_scale;
+    swm_grid_desc grid[4];   /* indexed by enum swm_grid_id */
+    const double *impl_u, *impl_v, *impl_eta;           /* implicit damping divisors */
+    const double *gamma_sq_u, *gamma_sq_v;              /* quadratic friction */
+    const double *F_x_const, *F_y_const, *F_eta_const;  /* constant forcing */
+    /* windstress streams re-applied every step as tau/(RHO0*Du) (FULLY_NONLINEAR,
+     * src/swm_forcing_module.f90:274-324,631-632); NULL = none */
+    const double *tau_x, *tau_y;
+    /* oscillating forcing per component (src/swm_forcing_module.f90:326-374);
+     * NULL = none.  Applied after windstress. */
+    const double *osc_re[3], *osc_im[3];
+    double osc_omega[3];
+} swm_desc;
+
+/* TRC_tracer, src/tracer_vars.f90:36-50 */
+typedef struct swm_tracer_desc {
+    double kappa_h;          /* horizontal diffusivity */
+    const double *CH;        /* (Nx,Ny) layer-integrated tracer at level N0 */
+    const double *C;         /* (Nx,Ny) concentration as held by the host (read by the next step) */
+    const double *G_CH;      /* (Nx,Ny,3) AB history, NULL = zero */
+    const double *C0;        /* restoring field, NULL = 0 */
+    const double *gamma_C;   /* relaxation coefficient, NULL = 0 */
+    const double *impl;      /* 1 + dt*cons, NULL = 1 */
+} swm_tracer_desc;
+
+typedef struct swm_ctx swm_ctx;
+
+/* Replaces SWM_initSWM / initCalcLib / init_ElSolv_SOR set-up of device data
+ * (src/swm_module.f90:37-48, src/calc_lib.f90:133-176, src/ElSolv_SOR.f90:44-66):
+ * uploads static fields, builds coefficient tables, allocates device state. */
+int swm_gpu_create(const swm_desc *d, swm_ctx **out);
+
+/* State hand-over after SWM_initialConditions (src/swm_module.f90:156-181):
+ * SWM_u/v/eta at N0 and G_u/G_v/G_eta (Nx,Ny,3) as held after SWM_advance.
+ * G pointers may be NULL (zero history).  itt is vars_module::itt. */
+int swm_gpu_upload_state(swm_ctx *c, const double *u, const double *v, const double *eta,
+                         const double *G_u, const double *G_v, const double *G_eta, int64_t itt);
+
+/* Replaces TRC_add_to_list (src/tracer_vars.f90:124-185) for one tracer.
+ * Tracers are stepped in the order they were added (TRC_timestep list order). */
+int swm_gpu_add_tracer(swm_ctx *c, const swm_tracer_desc *t, int32_t *tracer_id);
+
+/* nsteps iterations of the time loop body of src/model.f90:159-196 without Diag:
+ * SWM_timestep (src/swm_module.f90:71-74), TRC_timestep
+ * (src/tracer_module.f90:109-123) and model_advance (src/model.f90:238-259). */
+int swm_gpu_step(swm_ctx *c, int64_t nsteps);
+
+/* vars_module::itt as seen by the device state. */
+int64_t swm_gpu_itt(const swm_ctx *c);
+
+/* Copy one registered field to the host before a due diag task
+ * (src/diagTask.f90:266-303 reads them through the register,
+ * src/vars_module.f90:85-104).  Names are the reference's register names:
+ * SWM_U SWM_V SWM_ETA (also U V ETA), G_U G_V G_ETA (newest slot),
+ * SWM_D SWM_DH SWM_DU SWM_DV SWM_EDENS SWM_POT SWM_RELVORT SWM_MU SWM_MV
+ * F_X F_Y F_ETA SWM_LATMIX_U SWM_LATMIX_V, and for tracer_id >= 0:
+ * CH C G_CH uhc vhc.  Derived fields are materialised on demand from the
+ * device state.  Blocks. */
+int swm_gpu_sync(swm_ctx *c, const char *name, int32_t tracer_id, double *host_out);
+
+/* main_ElSolv_SOR(B, chi, epsilon, first_guess) contract
+ * (src/ElSolv_SOR.f90:150-222): red-black SOR with the reference's fixed
+ * Chebyshev omega schedule and RMS-of-correction stopping rule.  B and chi are
+ * host (Nx,Ny) arrays; chi is the first guess on entry and the result on exit.
+ * iters = sweeps used; anorm = final sqrt(sum res^2).  Returns 0 when
+ * converged, 1 when max_count sweeps did not converge (the reference only
+ * warns), other values on error. */
+int swm_gpu_sor(swm_ctx *c, const double *B, double *chi, double epsilon, int32_t first_guess,
+                int64_t *iters, double *anorm);
+
+/* Fixed-omega red-black SOR: stops when sqrt(sum r^2)/sqrt(sum B^2) < rel_tol
+ * after a full sweep, r = (L chi - B) at each cell when it is updated, or after
+ * max_sweeps.  fp32_sweeps != 0 runs the sweeps on FP32 copies of chi/B with
+ * the residual norm and convergence test in FP64.  Same return convention. */
+int swm_gpu_sor_fixed(swm_ctx *c, const double *B, double *chi, double omega, double rel_tol,
+                      int64_t max_sweeps, int32_t fp32_sweeps, int64_t *iters, double *rel_res);
+
+/* computeDivergence(u, v, div, u_grid, v_grid) of the current device velocity
+ * (src/calc_lib.f90:607-622), the right-hand side computeVelocityPotential
+ * hands to the SOR (src/calc_lib.f90:500-525).  Copies div to the host. */
+int swm_gpu_divergence(swm_ctx *c, double *host_div);
+
+/* Time spent in the last swm_gpu_step call's kernels (ms, device clock) and
+ * the average duration of the step kernel (ms), for roofline accounting. */
+int swm_gpu_last_timing(const swm_ctx *c, double *total_ms, double *step_kernel_ms);
+
+const char *swm_gpu_last_error(const swm_ctx *c);
+void swm_gpu_destroy(swm_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SWM_GPU_H */

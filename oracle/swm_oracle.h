@@ -1,0 +1,80 @@
+/*
+ * CPU oracle for the SWM time-stepping hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the parity checker and the CPU baseline.  It restates the
+ * reference Fortran pass for pass (same loop nests, same per-cell coefficient
+ * arrays and gather interpolation tables, same expression order, OpenMP
+ * schedule(static,5) over j).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  Nothing in the product links it.
+ *
+ * Arrays are the reference's Fortran (Nx,Ny) arrays viewed from C as [Ny][Nx]
+ * (i fastest), 0-based.  Masks are int8 0/1 ("ocean" masks; land = 1 - ocean).
+ */
+#ifndef SWM_ORACLE_H
+#define SWM_ORACLE_H
+#include <stdint.h>
+
+#define ORC_MAX_STREAMS 4
+#define ORC_MAX_TRACERS 4
+
+enum { ORC_GRID_H = 0, ORC_GRID_U = 1, ORC_GRID_V = 2, ORC_GRID_ETA = 3 };
+enum { ORC_FULLY_NONLINEAR = 1, ORC_LINEARISED_MEAN_STATE = 2, ORC_LINEARISED_STATE_OF_REST = 3 };
+enum { ORC_COMP_X = 0, ORC_COMP_Y = 1, ORC_COMP_ETA = 2 };
+
+typedef struct orc_grid {
+    const double *H;        /* (Nx,Ny) layer thickness on this grid, as left by initDomain/computeD */
+    const int8_t *ocean;    /* (Nx,Ny) ocean mask */
+    const double *cos_lat;  /* (Ny) */
+    const double *tan_lat;  /* (Ny) */
+    const double *f;        /* (Ny) Coriolis parameter */
+} orc_grid;
+
+typedef struct orc_desc {
+    int32_t Nx, Ny;
+    int32_t variant;             /* ORC_FULLY_NONLINEAR | ORC_LINEARISED_MEAN_STATE | ORC_LINEARISED_STATE_OF_REST */
+    int32_t lateral_mixing;      /* model.h LATERAL_MIXING */
+    int32_t quadratic_friction;  /* model.h QUADRATIC_BOTTOM_FRICTION */
+    int32_t barotropic;          /* model.h BAROTROPIC */
+    int32_t has_tau_scale;       /* model.h TAU_SCALE defined */
+    int32_t n_wind, n_osc, n_tracer;
+    double tau_scale;            /* value of the TAU_SCALE literal as the Fortran compiler sees it */
+    double G, Ah, dt, A, dLambda, dTheta, RHO0, minD, lbc, ab_chi;
+    orc_grid grid[4];            /* H, u, v, eta */
+    const double *impl_u, *impl_v, *impl_eta;
+    const double *gamma_sq_u, *gamma_sq_v;            /* only read with quadratic_friction */
+    const double *F_x_const, *F_y_const, *F_eta_const;
+    /* time-dependent windstress streams (constant-in-time data, re-applied every step) */
+    int32_t wind_comp[ORC_MAX_STREAMS];
+    const double *wind_tau[ORC_MAX_STREAMS];
+    /* oscillating forcing streams */
+    int32_t osc_comp[ORC_MAX_STREAMS];
+    const double *osc_re[ORC_MAX_STREAMS], *osc_im[ORC_MAX_STREAMS];
+    double osc_omega[ORC_MAX_STREAMS];
+    /* LINEARISED_MEAN_STATE basic state (H grid), NULL otherwise */
+    const double *u_bs, *v_bs, *zeta_bs;
+    /* initial conditions (unmasked; masked as in SWM_initialConditions) */
+    const double *u0, *v0, *eta0;
+    /* tracers */
+    double kappa_h[ORC_MAX_TRACERS];
+    const double *trc_C[ORC_MAX_TRACERS], *trc_C0[ORC_MAX_TRACERS];
+    const double *trc_gamma_C[ORC_MAX_TRACERS], *trc_cons[ORC_MAX_TRACERS];
+} orc_desc;
+
+typedef struct orc_model orc_model;
+
+orc_model *orc_create(const orc_desc *d);
+void orc_destroy(orc_model *m);
+/* advance nsteps time steps (SWM_timestep, TRC_timestep, model_advance) */
+void orc_step(orc_model *m, int64_t nsteps);
+int64_t orc_itt(const orc_model *m);
+/* copy a named field (register-like names) into out (Nx*Ny doubles); returns 0 on success */
+int orc_get(const orc_model *m, const char *name, int tracer, double *out);
+/* main_ElSolv_SOR contract: reference (Chebyshev) mode */
+int orc_sor(orc_model *m, const double *B, double *chi, double epsilon, int first_guess,
+            int64_t *iters, double *anorm);
+/* fixed-omega mode with relative-residual stopping rule */
+int orc_sor_fixed(orc_model *m, const double *B, double *chi, double omega, double rel_tol,
+                  int64_t max_sweeps, int64_t *iters, double *rel_res);
+/* computeDivergence(u, v, div, u_grid, v_grid) */
+void orc_divergence(const orc_model *m, const double *u, const double *v, double *div);
+#endif

@@ -1,0 +1,877 @@
+// C-ABI of the MI355X SWM time-stepping path (declarations: include/swm_gpu.h).
+//
+// Host side: uploads the static fields (compressing mask-row-uniform ones to two
+// values per row), builds per-row coefficient tables with the reference's
+// expression order in FP64 on the host, keeps u/v/eta (ping-pong) and the AB
+// history (3-slot ring: the reference's G shift becomes a role rotation) on the
+// device, and launches one fused step kernel per time step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "swm_gpu.h"
+#include "swm_kernels.h"
+
+using swm::Fld;
+using swm::StepArgs;
+
+namespace {
+
+constexpr int kTX = 64;   // step-kernel tile (outputs per block): 64 x 8
+constexpr int kTY = 8;
+
+template <typename T>
+struct FieldH {              // host description of an uploaded static field
+    T* full = nullptr;
+    T* rows = nullptr;
+};
+
+struct Tracer {
+    double kappa = 0;
+    void* CH[2] = {nullptr, nullptr};
+    void* G[3] = {nullptr, nullptr, nullptr};
+    void* Cbuf = nullptr;
+    bool cbuf_valid = false;
+    FieldH<double> C0d, gCd, impld;   // FP64 pointers
+    FieldH<float> C0f, gCf, implf;    // FP32 pointers
+};
+
+}  // namespace
+
+struct swm_ctx {
+    swm_desc d{};
+    int Nx = 0, Ny = 0, P = 0;
+    bool fp32 = false;
+    size_t esz = 8;
+    std::string err;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_total_ms = 0, last_kernel_ms = 0;
+    std::vector<void*> allocs;
+    // state
+    void* u[2] = {};
+    void* v[2] = {};
+    void* eta[2] = {};
+    int cur = 0;                 // ping-pong index of level N0
+    void* Gu[3] = {};
+    void* Gv[3] = {};
+    void* Ge[3] = {};
+    int ring = 0;                // slot holding G(1) (oldest); G(2) = ring+1, newest = ring+2
+    int64_t itt = 0;
+    uint8_t* mask = nullptr;
+    std::vector<int8_t> hmask[4];
+    // static fields and tables (device pointers, element type per precision)
+    void* fields[19][2] = {};    // [field][full|rows]
+    void* tab[32] = {};
+    double ab[5][5] = {};
+    std::vector<Tracer> trc;
+    // diagnostics scratch
+    void* diag[14] = {};
+    void* tdiag[2][3] = {};
+    void* scratch = nullptr;     // Ny*P elements (sync staging, divergence)
+    // SOR
+    void* sor_tab[4] = {};
+    void* sor_relax = nullptr;
+    int sor_nrelax = 0;
+    double* sor_partial = nullptr;
+    int* sor_done = nullptr;
+    long long* sor_iters = nullptr;
+    double* sor_norm = nullptr;
+    void* sor_buf[2][2] = {};    // [fp32][chi|B]
+    void* sor_ftab[2][4] = {};   // coefficient tables in the sweep precision
+    void* dza = nullptr;
+};
+
+namespace {
+
+enum FieldId {
+    F_HE, F_HU, F_HV, F_IMU, F_IMV, F_IME, F_GSU, F_GSV, F_FXC, F_FYC, F_FEC, F_TAUX, F_TAUY,
+    F_ORX, F_OIX, F_ORY, F_OIY, F_ORE, F_OIE, F_COUNT
+};
+enum TabId {
+    T_CU, T_CV, T_CE, T_CH, T_FH, T_ZA, T_EDU, T_ACE, T_LMU, T_LMV, T_PLLC, T_PLTC, T_TA, T_TB, T_TC,
+    T_T9, T_T11, T_T12, T_COUNT
+};
+
+int fail(swm_ctx* c, const std::string& msg, int code = 2) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                      \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail((ctx), std::string(#call) + ": " + hipGetErrorString(e_), 3);          \
+    } while (0)
+
+void* dalloc(swm_ctx* c, size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) return nullptr;
+    c->allocs.push_back(p);
+    return p;
+}
+
+// Upload a host double vector as the context's element type.
+void* upload_vec(swm_ctx* c, const std::vector<double>& h) {
+    void* p = dalloc(c, h.size() * c->esz);
+    if (!p) return nullptr;
+    if (c->fp32) {
+        std::vector<float> f(h.begin(), h.end());
+        if (hipMemcpy(p, f.data(), f.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    } else {
+        if (hipMemcpy(p, h.data(), h.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    }
+    return p;
+}
+
+// Upload a host (Nx,Ny) double array into a pitched device array of the context's type.
+int upload_grid(swm_ctx* c, void* dst, const double* src) {
+    const size_t n = (size_t)c->Nx * c->Ny;
+    if (c->fp32) {
+        std::vector<float> f(n);
+        for (size_t k = 0; k < n; ++k) f[k] = (float)src[k];
+        HIPCHK(c, hipMemcpy2D(dst, c->P * 4, f.data(), c->Nx * 4, c->Nx * 4, c->Ny, hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(c, hipMemcpy2D(dst, c->P * 8, src, c->Nx * 8, c->Nx * 8, c->Ny, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int download_grid(swm_ctx* c, double* dst, const void* src) {
+    const size_t n = (size_t)c->Nx * c->Ny;
+    if (c->fp32) {
+        std::vector<float> f(n);
+        HIPCHK(c, hipMemcpy2D(f.data(), c->Nx * 4, src, c->P * 4, c->Nx * 4, c->Ny, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < n; ++k) dst[k] = f[k];
+    } else {
+        HIPCHK(c, hipMemcpy2D(dst, c->Nx * 8, src, c->P * 8, c->Nx * 8, c->Ny, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+void* alloc_grid(swm_ctx* c) {
+    void* p = dalloc(c, (size_t)c->Ny * c->P * c->esz);
+    if (p && hipMemset(p, 0, (size_t)c->Ny * c->P * c->esz) != hipSuccess) return nullptr;
+    return p;
+}
+
+bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
+
+// Upload a static field.  NULL -> the constant default on every point.  If each
+// row has one value on the mask's ocean points and one on its land points the
+// field is stored as two values per row, else as a full pitched array.
+int upload_field(swm_ctx* c, int id, const double* h, double dflt, const int8_t* mask) {
+    const int Nx = c->Nx, Ny = c->Ny;
+    std::vector<double> rows(2 * (size_t)Ny, dflt);
+    bool compress = true;
+    if (h) {
+        for (int j = 0; j < Ny && compress; ++j) {
+            bool have[2] = {false, false};
+            for (int i = 0; i < Nx; ++i) {
+                const double x = h[(size_t)j * Nx + i];
+                const int s = mask[(size_t)j * Nx + i] ? 0 : 1;
+                if (!have[s]) { rows[2 * j + s] = x; have[s] = true; }
+                else if (!same_bits(rows[2 * j + s], x)) { compress = false; break; }
+            }
+        }
+    }
+    if (compress) {
+        c->fields[id][1] = upload_vec(c, rows);
+        if (!c->fields[id][1]) return fail(c, "field upload failed", 3);
+        c->fields[id][0] = nullptr;
+    } else {
+        void* p = alloc_grid(c);
+        if (!p) return fail(c, "out of device memory", 3);
+        int rc = upload_grid(c, p, h);
+        if (rc) return rc;
+        c->fields[id][0] = p;
+        c->fields[id][1] = nullptr;
+    }
+    return 0;
+}
+
+template <typename T>
+Fld<T> F(const swm_ctx* c, int id) {
+    return Fld<T>{(const T*)c->fields[id][0], (const T*)c->fields[id][1]};
+}
+
+// Per-row coefficient tables, each built with the reference's expression.
+int build_tables(swm_ctx* c) {
+    const swm_desc& d = c->d;
+    const int Ny = c->Ny;
+    const double A = d.A, dL = d.dLambda, dT = d.dTheta, Ah = d.Ah;
+    const double *cu = d.grid[SWM_GRID_U].cos_lat, *cv = d.grid[SWM_GRID_V].cos_lat,
+                 *ce = d.grid[SWM_GRID_ETA].cos_lat, *cH = d.grid[SWM_GRID_H].cos_lat;
+    const double *tu = d.grid[SWM_GRID_U].tan_lat, *tv = d.grid[SWM_GRID_V].tan_lat,
+                 *te = d.grid[SWM_GRID_ETA].tan_lat, *tH = d.grid[SWM_GRID_H].tan_lat;
+    std::vector<double> t[T_COUNT];
+    t[T_CU].assign(cu, cu + Ny);
+    t[T_CV].assign(cv, cv + Ny);
+    t[T_CE].assign(ce, ce + Ny);
+    t[T_CH].assign(cH, cH + Ny);
+    t[T_FH].assign(d.grid[SWM_GRID_H].f, d.grid[SWM_GRID_H].f + Ny);
+    t[T_ZA].resize(2 * Ny); t[T_EDU].resize(Ny); t[T_ACE].resize(Ny);
+    t[T_LMU].resize(6 * Ny); t[T_LMV].resize(6 * Ny); t[T_PLLC].resize(12 * Ny); t[T_PLTC].resize(12 * Ny);
+    t[T_TA].resize(2 * Ny); t[T_TB].resize(2 * Ny); t[T_TC].resize(2 * Ny);
+    t[T_T9].resize(Ny); t[T_T11].resize(Ny); t[T_T12].resize(Ny);
+    for (int j = 0; j < Ny; ++j) {
+        const int jp = (j + 1) % Ny;
+        // pder_zonal3D factor bc/(A*cos*dLambda) on the H grid (src/calc_lib.f90:439)
+        t[T_ZA][2 * j + 0] = 1.0 / (A * cH[j] * dL);
+        t[T_ZA][2 * j + 1] = d.lbc / (A * cH[j] * dL);
+        t[T_EDU][j] = A * cu[j] * dL;
+        t[T_ACE][j] = A * ce[j];
+        for (int ot = 1; ot <= 2; ++ot) {
+            // SWM_LateralMixing_init (src/swm_lateralmixing_module.f90:88-94), bc = 1
+            const double bc = 1.0;
+            double* lu = &t[T_LMU][(j * 2 + ot - 1) * 3];
+            lu[0] = -(bc / A / cu[j] / dL);
+            lu[1] = -(bc / A / dT - 2.0 * tu[j] / A / (double)ot);
+            lu[2] = -(-(bc / A / dT) - 2.0 * tu[j] / A / (double)ot);
+            double* lv = &t[T_LMV][(j * 2 + ot - 1) * 3];
+            lv[0] = -(bc / A / cv[j] / dL);
+            lv[1] = -(-(bc / A / dT) + 2.0 * tv[j] / A / (double)ot);
+            lv[2] = -(bc / A / dT + 2.0 * tv[j] / A / (double)ot);
+        }
+        for (int a1 = 0; a1 < 2; ++a1)
+            for (int a0 = 0; a0 < 2; ++a0) {
+                // _init_p_coefficients (src/swm_lateralmixing_module.f90:135-142), bc = 1
+                const double bc = 1.0;
+                int ot = std::max(1, a0 + a1);
+                double* pl = &t[T_PLLC][(j * 4 + a1 * 2 + a0) * 3];   // a1 = ov(j+1), a0 = ov(j)
+                pl[0] = -(Ah * bc / dL / A / ce[j]);
+                pl[1] = Ah / A * (bc / dT + (double)a1 * te[j] / (double)ot);
+                pl[2] = Ah / A * (-(bc / dT) + (double)a0 * te[j] / (double)ot);
+                double* pt = &t[T_PLTC][(j * 4 + a1 * 2 + a0) * 3];   // a1 = ou(j), a0 = ou(j-1)
+                pt[0] = -(Ah * bc / dL / A / cH[j]);
+                pt[1] = -(Ah / A * (bc / dT + (double)a1 * tH[j] / (double)ot));
+                pt[2] = -(Ah / A * (-(bc / dT) + (double)a0 * tH[j] / (double)ot));
+            }
+        // TRC_initCoeffs magnitudes (src/tracer_module.f90:318-338)
+        for (int den = 1; den <= 2; ++den) {
+            t[T_TA][2 * j + den - 1] = 1.0 * 1.0 * 1.0 / (double)den / A / dL / ce[j];
+            t[T_TB][2 * j + den - 1] = 1.0 * 1.0 * cv[jp] * 1.0 / (double)den / A / dT / ce[j];
+            t[T_TC][2 * j + den - 1] = 1.0 * 1.0 * cv[j] * 1.0 / (double)den / A / dT / ce[j];
+        }
+        t[T_T9][j] = 1.0 * 1.0 / (dL * dL) / (A * A) / cu[j] / ce[j];
+        t[T_T11][j] = 1.0 * 1.0 * cv[jp] / (dT * dT) / (A * A) / ce[j];
+        t[T_T12][j] = 1.0 * 1.0 * cv[j] / (dT * dT) / (A * A) / ce[j];
+    }
+    for (int k = 0; k < T_COUNT; ++k) {
+        c->tab[k] = upload_vec(c, t[k]);
+        if (!c->tab[k]) return fail(c, "table upload failed", 3);
+    }
+    // computeDivergence zonal factor on the eta grid
+    std::vector<double> dza(2 * Ny);
+    for (int j = 0; j < Ny; ++j) {
+        dza[2 * j] = 1.0 / (A * ce[j] * dL);
+        dza[2 * j + 1] = d.lbc / (A * ce[j] * dL);
+    }
+    c->dza = upload_vec(c, dza);
+    // SOR coefficients (src/ElSolv_SOR.f90:54-62)
+    std::vector<double> c1(Ny), c3(Ny), c4(Ny), c5(12 * (size_t)Ny);
+    for (int j = 0; j < Ny; ++j) {
+        const int jp = (j + 1) % Ny;
+        double x = A * ce[j] * dL;
+        x = x * x;
+        const double y = A * A * ce[j] * (dT * dT);
+        c1[j] = 1.0 / x;
+        c3[j] = cv[jp] * 1.0 / y;
+        c4[j] = cv[j] * 1.0 / y;
+        for (int su = 0; su <= 2; ++su)
+            for (int ovjp = 0; ovjp < 2; ++ovjp)
+                for (int ov = 0; ov < 2; ++ov)
+                    c5[((j * 3 + su) * 2 + ovjp) * 2 + ov] =
+                        (double)(-su) / x + (-(cv[jp] * (double)ovjp) - cv[j] * (double)ov) / y;
+    }
+    c->sor_tab[0] = upload_vec(c, c1);
+    c->sor_tab[1] = upload_vec(c, c3);
+    c->sor_tab[2] = upload_vec(c, c4);
+    c->sor_tab[3] = upload_vec(c, c5);
+    if (!c->dza || !c->sor_tab[0] || !c->sor_tab[1] || !c->sor_tab[2] || !c->sor_tab[3])
+        return fail(c, "table upload failed", 3);
+    return 0;
+}
+
+template <typename T>
+StepArgs<T> make_args(swm_ctx* c) {
+    const swm_desc& d = c->d;
+    StepArgs<T> a{};
+    a.Nx = c->Nx; a.Ny = c->Ny; a.P = c->P;
+    a.ntx = (c->Nx + kTX - 1) / kTX;
+    a.nty = (c->Ny + kTY - 1) / kTY;
+    a.nblk = a.ntx * a.nty;
+    a.mix = (d.flags & SWM_LATERAL_MIXING) ? 1 : 0;
+    a.quad = (d.flags & SWM_QUADRATIC_BOTTOM_FRICTION) ? 1 : 0;
+    a.barotropic = (d.flags & SWM_BAROTROPIC) ? 1 : 0;
+    a.has_tau_scale = (d.flags & SWM_TAU_SCALE) ? 1 : 0;
+    a.has_taux = d.tau_x != nullptr;
+    a.has_tauy = d.tau_y != nullptr;
+    a.has_ox = d.osc_re[0] != nullptr;
+    a.has_oy = d.osc_re[1] != nullptr;
+    a.has_oe = d.osc_re[2] != nullptr;
+    a.G = (T)d.G; a.A = (T)d.A; a.dL = (T)d.dLambda; a.dT = (T)d.dTheta; a.RHO0 = (T)d.RHO0;
+    a.minD = (T)d.minD; a.lbc = (T)d.lbc; a.tau_scale = (T)d.tau_scale; a.AdT = (T)(d.A * d.dTheta);
+    a.eps = std::is_same<T, double>::value ? (T)2.220446049250313e-16 : (T)1.1920929e-07f;
+    a.mask = c->mask;
+    a.He = F<T>(c, F_HE); a.Hu = F<T>(c, F_HU); a.Hv = F<T>(c, F_HV);
+    a.imu = F<T>(c, F_IMU); a.imv = F<T>(c, F_IMV); a.ime = F<T>(c, F_IME);
+    a.gsu = F<T>(c, F_GSU); a.gsv = F<T>(c, F_GSV);
+    a.Fxc = F<T>(c, F_FXC); a.Fyc = F<T>(c, F_FYC); a.Fec = F<T>(c, F_FEC);
+    a.taux = F<T>(c, F_TAUX); a.tauy = F<T>(c, F_TAUY);
+    a.oRx = F<T>(c, F_ORX); a.oIx = F<T>(c, F_OIX); a.oRy = F<T>(c, F_ORY); a.oIy = F<T>(c, F_OIY);
+    a.oRe = F<T>(c, F_ORE); a.oIe = F<T>(c, F_OIE);
+    a.cu = (const T*)c->tab[T_CU]; a.cv = (const T*)c->tab[T_CV]; a.ce = (const T*)c->tab[T_CE];
+    a.cH = (const T*)c->tab[T_CH]; a.fH = (const T*)c->tab[T_FH]; a.za = (const T*)c->tab[T_ZA];
+    a.edu = (const T*)c->tab[T_EDU]; a.ace = (const T*)c->tab[T_ACE];
+    a.lmu = (const T*)c->tab[T_LMU]; a.lmv = (const T*)c->tab[T_LMV];
+    a.pllc = (const T*)c->tab[T_PLLC]; a.pltc = (const T*)c->tab[T_PLTC];
+    a.tA = (const T*)c->tab[T_TA]; a.tB = (const T*)c->tab[T_TB]; a.tC = (const T*)c->tab[T_TC];
+    a.t9 = (const T*)c->tab[T_T9]; a.t11 = (const T*)c->tab[T_T11]; a.t12 = (const T*)c->tab[T_T12];
+    // state at N0 and the AB roles
+    const int n0 = c->cur, n1 = 1 - c->cur;
+    const int r0 = c->ring, r1 = (c->ring + 1) % 3, r2 = (c->ring + 2) % 3;
+    a.u = (const T*)c->u[n0]; a.v = (const T*)c->v[n0]; a.eta = (const T*)c->eta[n0];
+    a.u_n = (T*)c->u[n1]; a.v_n = (T*)c->v[n1]; a.eta_n = (T*)c->eta[n1];
+    a.Gu1 = (const T*)c->Gu[r0]; a.Gu2 = (const T*)c->Gu[r1]; a.Gu3 = (T*)c->Gu[r2];
+    a.Gv1 = (const T*)c->Gv[r0]; a.Gv2 = (const T*)c->Gv[r1]; a.Gv3 = (T*)c->Gv[r2];
+    a.Ge1 = (const T*)c->Ge[r0]; a.Ge2 = (const T*)c->Ge[r1]; a.Ge3 = (T*)c->Ge[r2];
+    for (size_t n = 0; n < c->trc.size() && n < 2; ++n) {
+        Tracer& tr = c->trc[n];
+        swm::TrcArgs<T>& ta = a.trc[n];
+        ta.kappa = (T)tr.kappa;
+        ta.CH = (const T*)tr.CH[n0]; ta.CH_new = (T*)tr.CH[n1];
+        ta.Cbuf = tr.cbuf_valid ? (const T*)tr.Cbuf : nullptr;
+        ta.G1 = (const T*)tr.G[r0]; ta.G2 = (const T*)tr.G[r1]; ta.G3 = (T*)tr.G[r2];
+        if (std::is_same<T, double>::value) {
+            ta.C0 = Fld<T>{(const T*)tr.C0d.full, (const T*)tr.C0d.rows};
+            ta.gC = Fld<T>{(const T*)tr.gCd.full, (const T*)tr.gCd.rows};
+            ta.impl = Fld<T>{(const T*)tr.impld.full, (const T*)tr.impld.rows};
+        } else {
+            ta.C0 = Fld<T>{(const T*)tr.C0f.full, (const T*)tr.C0f.rows};
+            ta.gC = Fld<T>{(const T*)tr.gCf.full, (const T*)tr.gCf.rows};
+            ta.impl = Fld<T>{(const T*)tr.implf.full, (const T*)tr.implf.rows};
+        }
+        ta.dC = (T*)c->tdiag[n][0]; ta.duhc = (T*)c->tdiag[n][1]; ta.dvhc = (T*)c->tdiag[n][2];
+    }
+    return a;
+}
+
+template <typename T, bool DIAG>
+int launch_step_t(swm_ctx* c, const StepArgs<T>& a) {
+    const int var = c->d.variant;
+    const int nt = (int)c->trc.size();
+    dim3 grid(a.nblk), block(256);
+#define SWM_LAUNCH(V, N)                                                                                   \
+    hipLaunchKernelGGL((swm::step_kernel<T, V, N, kTX, kTY, DIAG>), grid, block, 0, c->stream, a)
+    if (var == SWM_FULLY_NONLINEAR) {
+        if (nt == 0) SWM_LAUNCH(swm::VAR_FN, 0);
+        else if (nt == 1) SWM_LAUNCH(swm::VAR_FN, 1);
+        else SWM_LAUNCH(swm::VAR_FN, 2);
+    } else {
+        if (nt == 0) SWM_LAUNCH(swm::VAR_SOR, 0);
+        else if (nt == 1) SWM_LAUNCH(swm::VAR_SOR, 1);
+        else SWM_LAUNCH(swm::VAR_SOR, 2);
+    }
+#undef SWM_LAUNCH
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int step_one(swm_ctx* c) {
+    c->itt += 1;
+    StepArgs<T> a = make_args<T>(c);
+    const int tstep = (int)std::min<int64_t>(3, c->itt);
+    a.tstep = tstep;
+    a.dtc1 = (T)(c->d.dt * c->ab[tstep - 1][0]);
+    a.dtc2 = (T)(c->d.dt * c->ab[tstep - 1][1]);
+    a.dtc3 = (T)(c->d.dt * c->ab[tstep - 1][2]);
+    // the forcing used by step itt was set by SWM_forcing_update at itt-1
+    const double tf = (double)(c->itt - 1);
+    for (int comp = 0; comp < 3; ++comp) {
+        if (!c->d.osc_re[comp]) continue;
+        const double arg = c->d.osc_omega[comp] * tf * c->d.dt;
+        const double cs = std::cos(arg), sn = std::sin(arg);
+        if (comp == 0) { a.ocx = (T)cs; a.osx = (T)sn; }
+        if (comp == 1) { a.ocy = (T)cs; a.osy = (T)sn; }
+        if (comp == 2) { a.oce = (T)cs; a.ose = (T)sn; }
+    }
+    int rc = launch_step_t<T, false>(c, a);
+    if (rc) return rc;
+    // model_advance: N0 <- N0p1, G(1) <- G(2), G(2) <- G(3)
+    c->cur = 1 - c->cur;
+    c->ring = (c->ring + 1) % 3;
+    for (auto& tr : c->trc) tr.cbuf_valid = false;
+    return 0;
+}
+
+template <typename T>
+int materialise(swm_ctx* c) {
+    for (int k = 0; k < 14; ++k)
+        if (!c->diag[k] && !(c->diag[k] = alloc_grid(c))) return fail(c, "out of device memory", 3);
+    for (size_t n = 0; n < c->trc.size(); ++n)
+        for (int k = 0; k < 3; ++k)
+            if (!c->tdiag[n][k] && !(c->tdiag[n][k] = alloc_grid(c))) return fail(c, "out of device memory", 3);
+    StepArgs<T> a = make_args<T>(c);
+    T** outs[14] = {&a.dD, &a.dDh, &a.dDu, &a.dDv, &a.dzeta, &a.dPot, &a.dEd, &a.dMU, &a.dMV,
+                    &a.dFx, &a.dFy, &a.dFe, &a.dLu, &a.dLv};
+    for (int k = 0; k < 14; ++k) *outs[k] = (T*)c->diag[k];
+    a.tstep = 1;
+    const double tf = (double)c->itt;   // forcing as left by the last SWM_forcing_update
+    for (int comp = 0; comp < 3; ++comp) {
+        if (!c->d.osc_re[comp]) continue;
+        const double arg = c->d.osc_omega[comp] * tf * c->d.dt;
+        const double cs = std::cos(arg), sn = std::sin(arg);
+        if (comp == 0) { a.ocx = (T)cs; a.osx = (T)sn; }
+        if (comp == 1) { a.ocy = (T)cs; a.osy = (T)sn; }
+        if (comp == 2) { a.oce = (T)cs; a.ose = (T)sn; }
+    }
+    return launch_step_t<T, true>(c, a);
+}
+
+}  // namespace
+
+extern "C" {
+
+static thread_local std::string g_create_err;
+
+static int create_impl(swm_ctx* c, const swm_desc* d);
+
+int swm_gpu_create(const swm_desc* d, swm_ctx** out) {
+    if (!out) return 1;
+    *out = nullptr;
+    g_create_err.clear();
+    if (!d) { g_create_err = "null descriptor"; return 1; }
+    if (d->abi_version != SWM_GPU_ABI_VERSION) { g_create_err = "ABI version mismatch"; return 1; }
+    if (d->Nx < 3 || d->Ny < 3) { g_create_err = "Nx and Ny must be >= 3"; return 1; }
+    if (d->variant != SWM_FULLY_NONLINEAR && d->variant != SWM_LINEARISED_STATE_OF_REST) {
+        g_create_err = "variant not supported on the GPU path (FULLY_NONLINEAR, LINEARISED_STATE_OF_REST)";
+        return 4;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_create_err = "no HIP device"; return 5; }
+    swm_ctx* c = new swm_ctx();
+    const int rc = create_impl(c, d);
+    if (rc) {
+        g_create_err = c->err.empty() ? "create failed" : c->err;
+        swm_gpu_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+static int create_impl(swm_ctx* c, const swm_desc* d) {
+    c->d = *d;
+    c->Nx = d->Nx;
+    c->Ny = d->Ny;
+    c->fp32 = d->precision == SWM_FP32;
+    c->esz = c->fp32 ? 4 : 8;
+    const int align = c->fp32 ? 32 : 16;   // 128-byte aligned rows
+    c->P = (c->Nx + align - 1) / align * align;
+    for (int g = 0; g < 4; ++g) {
+        if (!d->grid[g].ocean || !d->grid[g].cos_lat || !d->grid[g].tan_lat || !d->grid[g].f)
+            return fail(c, "every grid needs ocean, cos_lat, tan_lat and f", 1);
+        c->hmask[g].assign(d->grid[g].ocean, d->grid[g].ocean + (size_t)c->Nx * c->Ny);
+    }
+    HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreate(&c->ev0));
+    HIPCHK(c, hipEventCreate(&c->ev1));
+    // time_integration_init (src/time_integration_module.f90:44-51)
+    const double chi = d->ab_chi;
+    c->ab[0][0] = 1.0;
+    c->ab[1][0] = -0.5 - chi; c->ab[1][1] = 1.5 + chi;
+    c->ab[2][0] = 5.0 / 12.0; c->ab[2][1] = -16.0 / 12.0; c->ab[2][2] = 23.0 / 12.0;
+    // packed masks: bit0 H, bit1 u, bit2 v, bit3 eta
+    {
+        const size_t n = (size_t)c->Nx * c->Ny;
+        std::vector<uint8_t> m((size_t)c->Ny * c->P, 0);
+        for (int j = 0; j < c->Ny; ++j)
+            for (int i = 0; i < c->Nx; ++i) {
+                const size_t k = (size_t)j * c->Nx + i;
+                m[(size_t)j * c->P + i] = (uint8_t)((c->hmask[0][k] ? 1 : 0) | (c->hmask[1][k] ? 2 : 0) |
+                                                    (c->hmask[2][k] ? 4 : 0) | (c->hmask[3][k] ? 8 : 0));
+            }
+        (void)n;
+        c->mask = (uint8_t*)dalloc(c, m.size());
+        if (!c->mask) return fail(c, "out of device memory", 3);
+        HIPCHK(c, hipMemcpy(c->mask, m.data(), m.size(), hipMemcpyHostToDevice));
+    }
+    const int8_t *mH = c->hmask[0].data(), *mu = c->hmask[1].data(), *mv = c->hmask[2].data(),
+                 *me = c->hmask[3].data();
+    (void)mH;
+    int rc = 0;
+    if (!d->grid[SWM_GRID_ETA].H) return fail(c, "grid[ETA].H is required", 1);
+    if ((rc = upload_field(c, F_HE, d->grid[SWM_GRID_ETA].H, 0.0, me))) return rc;
+    if ((rc = upload_field(c, F_HU, d->grid[SWM_GRID_U].H, 0.0, mu))) return rc;
+    if ((rc = upload_field(c, F_HV, d->grid[SWM_GRID_V].H, 0.0, mv))) return rc;
+    if ((rc = upload_field(c, F_IMU, d->impl_u, 1.0, mu))) return rc;
+    if ((rc = upload_field(c, F_IMV, d->impl_v, 1.0, mv))) return rc;
+    if ((rc = upload_field(c, F_IME, d->impl_eta, 1.0, me))) return rc;
+    if ((rc = upload_field(c, F_GSU, d->gamma_sq_u, 0.0, mu))) return rc;
+    if ((rc = upload_field(c, F_GSV, d->gamma_sq_v, 0.0, mv))) return rc;
+    if ((rc = upload_field(c, F_FXC, d->F_x_const, 0.0, mu))) return rc;
+    if ((rc = upload_field(c, F_FYC, d->F_y_const, 0.0, mv))) return rc;
+    if ((rc = upload_field(c, F_FEC, d->F_eta_const, 0.0, me))) return rc;
+    if ((rc = upload_field(c, F_TAUX, d->tau_x, 0.0, mu))) return rc;
+    if ((rc = upload_field(c, F_TAUY, d->tau_y, 0.0, mv))) return rc;
+    if ((rc = upload_field(c, F_ORX, d->osc_re[0], 0.0, mu))) return rc;
+    if ((rc = upload_field(c, F_OIX, d->osc_im[0], 0.0, mu))) return rc;
+    if ((rc = upload_field(c, F_ORY, d->osc_re[1], 0.0, mv))) return rc;
+    if ((rc = upload_field(c, F_OIY, d->osc_im[1], 0.0, mv))) return rc;
+    if ((rc = upload_field(c, F_ORE, d->osc_re[2], 0.0, me))) return rc;
+    if ((rc = upload_field(c, F_OIE, d->osc_im[2], 0.0, me))) return rc;
+    if ((d->osc_re[0] && !d->osc_im[0]) || (d->osc_re[1] && !d->osc_im[1]) || (d->osc_re[2] && !d->osc_im[2]))
+        return fail(c, "oscillating forcing needs both real and imaginary parts", 1);
+    if ((d->tau_x || d->tau_y) && d->variant != SWM_FULLY_NONLINEAR)
+        return fail(c, "per-step windstress streams are a FULLY_NONLINEAR feature; fold constant "
+                       "windstress into F_*_const for linear variants", 1);
+    if ((rc = build_tables(c))) return rc;
+    for (int l = 0; l < 2; ++l) {
+        if (!(c->u[l] = alloc_grid(c)) || !(c->v[l] = alloc_grid(c)) || !(c->eta[l] = alloc_grid(c)))
+            return fail(c, "out of device memory", 3);
+    }
+    for (int l = 0; l < 3; ++l) {
+        if (!(c->Gu[l] = alloc_grid(c)) || !(c->Gv[l] = alloc_grid(c)) || !(c->Ge[l] = alloc_grid(c)))
+            return fail(c, "out of device memory", 3);
+    }
+    if (!(c->scratch = alloc_grid(c))) return fail(c, "out of device memory", 3);
+    return 0;
+}
+
+int swm_gpu_upload_state(swm_ctx* c, const double* u, const double* v, const double* eta, const double* G_u,
+                         const double* G_v, const double* G_eta, int64_t itt) {
+    if (!c) return 1;
+    if (!u || !v || !eta) return fail(c, "u, v and eta are required", 1);
+    if (itt < 0) return fail(c, "itt must be >= 0", 1);
+    const size_t n = (size_t)c->Nx * c->Ny;
+    c->cur = 0;
+    c->ring = 0;
+    int rc;
+    if ((rc = upload_grid(c, c->u[0], u)) || (rc = upload_grid(c, c->v[0], v)) || (rc = upload_grid(c, c->eta[0], eta)))
+        return rc;
+    std::vector<double> zero(n, 0.0);
+    for (int l = 0; l < 3; ++l) {
+        if ((rc = upload_grid(c, c->Gu[l], G_u ? G_u + l * n : zero.data()))) return rc;
+        if ((rc = upload_grid(c, c->Gv[l], G_v ? G_v + l * n : zero.data()))) return rc;
+        if ((rc = upload_grid(c, c->Ge[l], G_eta ? G_eta + l * n : zero.data()))) return rc;
+    }
+    c->itt = itt;
+    return 0;
+}
+
+int swm_gpu_add_tracer(swm_ctx* c, const swm_tracer_desc* t, int32_t* tracer_id) {
+    if (!c || !t) return 1;
+    if (c->trc.size() >= 2) return fail(c, "at most 2 tracers are fused into the step kernel", 4);
+    if (!t->CH || !t->C) return fail(c, "tracer CH and C are required", 1);
+    if (c->d.variant != SWM_FULLY_NONLINEAR)
+        return fail(c, "tracers need FULLY_NONLINEAR (h = D); the linear D alias makes C = CH/0 on land", 4);
+    const size_t n = (size_t)c->Nx * c->Ny;
+    Tracer tr;
+    tr.kappa = t->kappa_h;
+    int rc;
+    for (int l = 0; l < 2; ++l)
+        if (!(tr.CH[l] = alloc_grid(c))) return fail(c, "out of device memory", 3);
+    for (int l = 0; l < 3; ++l)
+        if (!(tr.G[l] = alloc_grid(c))) return fail(c, "out of device memory", 3);
+    if (!(tr.Cbuf = alloc_grid(c))) return fail(c, "out of device memory", 3);
+    // the tracer's buffers follow the context's current ping-pong / ring roles
+    if ((rc = upload_grid(c, tr.CH[c->cur], t->CH))) return rc;
+    if ((rc = upload_grid(c, tr.Cbuf, t->C))) return rc;
+    std::vector<double> zero(n, 0.0);
+    for (int l = 0; l < 3; ++l)
+        if ((rc = upload_grid(c, tr.G[(c->ring + l) % 3], t->G_CH ? t->G_CH + l * n : zero.data()))) return rc;
+    tr.cbuf_valid = true;
+    const int8_t* me = c->hmask[3].data();
+    const int F0 = F_COUNT;   // temporary slot in fields[] is not used: upload directly
+    (void)F0;
+    auto up = [&](const double* h, double dflt, void** full, void** rows) -> int {
+        void* save[2] = {c->fields[0][0], c->fields[0][1]};
+        int r = upload_field(c, 0, h, dflt, me);
+        *full = c->fields[0][0];
+        *rows = c->fields[0][1];
+        c->fields[0][0] = save[0];
+        c->fields[0][1] = save[1];
+        return r;
+    };
+    void *f0, *r0, *f1, *r1, *f2, *r2;
+    if ((rc = up(t->C0, 0.0, &f0, &r0)) || (rc = up(t->gamma_C, 0.0, &f1, &r1)) || (rc = up(t->impl, 1.0, &f2, &r2)))
+        return rc;
+    if (c->fp32) {
+        tr.C0f = {(float*)f0, (float*)r0}; tr.gCf = {(float*)f1, (float*)r1}; tr.implf = {(float*)f2, (float*)r2};
+    } else {
+        tr.C0d = {(double*)f0, (double*)r0}; tr.gCd = {(double*)f1, (double*)r1}; tr.impld = {(double*)f2, (double*)r2};
+    }
+    c->trc.push_back(tr);
+    if (tracer_id) *tracer_id = (int32_t)c->trc.size() - 1;
+    return 0;
+}
+
+int swm_gpu_step(swm_ctx* c, int64_t nsteps) {
+    if (!c) return 1;
+    if (nsteps < 0) return fail(c, "nsteps must be >= 0", 1);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int64_t s = 0; s < nsteps; ++s) {
+        int rc = c->fp32 ? step_one<float>(c) : step_one<double>(c);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_total_ms = ms;
+    c->last_kernel_ms = nsteps > 0 ? ms / (double)nsteps : 0.0;
+    return 0;
+}
+
+int64_t swm_gpu_itt(const swm_ctx* c) { return c ? c->itt : -1; }
+
+int swm_gpu_last_timing(const swm_ctx* c, double* total_ms, double* step_kernel_ms) {
+    if (!c) return 1;
+    if (total_ms) *total_ms = c->last_total_ms;
+    if (step_kernel_ms) *step_kernel_ms = c->last_kernel_ms;
+    return 0;
+}
+
+int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_out) {
+    if (!c || !name || !host_out) return 1;
+    std::string s(name);
+    for (auto& ch : s) ch = (char)toupper((unsigned char)ch);
+    const int n0 = c->cur;
+    const int r1 = (c->ring + 1) % 3, r0 = c->ring;
+    const void* src = nullptr;
+    if (tracer_id >= 0) {
+        if (tracer_id >= (int)c->trc.size()) return fail(c, "no such tracer", 1);
+        Tracer& tr = c->trc[tracer_id];
+        if (s == "CH") src = tr.CH[n0];
+        else if (s == "G_CH") src = tr.G[r1];      // newest tendency (G(3) == G(2) after the shift)
+        else if (s == "G_CH_M1") src = tr.G[r0];
+        else if (s == "C" || s == "UHC" || s == "VHC") {
+            int rc = c->fp32 ? materialise<float>(c) : materialise<double>(c);
+            if (rc) return rc;
+            src = c->tdiag[tracer_id][s == "C" ? 0 : (s == "UHC" ? 1 : 2)];
+        } else {
+            return fail(c, "unknown tracer field " + s, 1);
+        }
+    } else if (s == "SWM_U" || s == "U") src = c->u[n0];
+    else if (s == "SWM_V" || s == "V") src = c->v[n0];
+    else if (s == "SWM_ETA" || s == "ETA") src = c->eta[n0];
+    else if (s == "G_U") src = c->Gu[r1];
+    else if (s == "G_V") src = c->Gv[r1];
+    else if (s == "G_ETA") src = c->Ge[r1];
+    else if (s == "G_U_M1") src = c->Gu[r0];
+    else if (s == "G_V_M1") src = c->Gv[r0];
+    else if (s == "G_ETA_M1") src = c->Ge[r0];
+    else {
+        static const char* names[14] = {"SWM_D", "SWM_DH", "SWM_DU", "SWM_DV", "SWM_RELVORT", "SWM_POT", "SWM_EDENS",
+                                        "SWM_MU", "SWM_MV", "F_X", "F_Y", "F_ETA", "SWM_LATMIX_U", "SWM_LATMIX_V"};
+        int k = -1;
+        for (int q = 0; q < 14; ++q)
+            if (s == names[q]) k = q;
+        if (k < 0) return fail(c, "unknown field " + s, 1);
+        int rc = c->fp32 ? materialise<float>(c) : materialise<double>(c);
+        if (rc) return rc;
+        src = c->diag[k];
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return download_grid(c, host_out, src);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Device buffers of the SOR driver, allocated on first use and kept: chi/B in
+// the sweep precision, coefficient tables in that precision, the omega schedule.
+int sor_prepare(swm_ctx* c, bool f32, int nblk) {
+    if (!c->sor_partial) {
+        c->sor_partial = (double*)dalloc(c, sizeof(double) * 2 * (size_t)nblk);
+        c->sor_done = (int*)dalloc(c, sizeof(int));
+        c->sor_iters = (long long*)dalloc(c, sizeof(long long));
+        c->sor_norm = (double*)dalloc(c, sizeof(double));
+        if (!c->sor_partial || !c->sor_done || !c->sor_iters || !c->sor_norm)
+            return fail(c, "out of device memory", 3);
+    }
+    const int w = f32 ? 1 : 0;
+    const size_t bytes = (size_t)c->P * c->Ny * (f32 ? 4 : 8);
+    if (!c->sor_buf[w][0]) {
+        c->sor_buf[w][0] = dalloc(c, bytes);
+        c->sor_buf[w][1] = dalloc(c, bytes);
+        if (!c->sor_buf[w][0] || !c->sor_buf[w][1]) return fail(c, "out of device memory", 3);
+    }
+    if (!c->sor_ftab[w][0]) {
+        const int Ny = c->Ny;
+        const size_t sz[4] = {(size_t)Ny, (size_t)Ny, (size_t)Ny, 12 * (size_t)Ny};
+        for (int q = 0; q < 4; ++q) {
+            if (f32 == c->fp32) { c->sor_ftab[w][q] = c->sor_tab[q]; continue; }
+            // tables were uploaded in the context precision; convert on the host
+            std::vector<double> h(sz[q]);
+            std::vector<float> f(sz[q]);
+            HIPCHK(c, hipMemcpy(h.data(), c->sor_tab[q], sz[q] * 8, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < sz[q]; ++k) f[k] = (float)h[k];
+            if (!(c->sor_ftab[w][q] = dalloc(c, sz[q] * 4))) return fail(c, "out of device memory", 3);
+            HIPCHK(c, hipMemcpy(c->sor_ftab[w][q], f.data(), sz[q] * 4, hipMemcpyHostToDevice));
+        }
+    }
+    if (!c->sor_relax) {
+        // reference omega schedule (src/ElSolv_SOR.f90:175-210), kept until stationary
+        const double rJ = (double)9.9999178e-1f;
+        std::vector<double> relax;
+        double wv = 1.0;
+        relax.push_back(wv);
+        for (int h = 1; h < (1 << 20); ++h) {
+            const double nw = (h == 1) ? 1.0 / (1.0 - .5 * (rJ * rJ)) : 1.0 / (1.0 - .25 * (rJ * rJ) * wv);
+            relax.push_back(nw);
+            if (h > 2 && nw == wv) break;
+            wv = nw;
+        }
+        if (!(c->sor_relax = dalloc(c, relax.size() * 8))) return fail(c, "out of device memory", 3);
+        HIPCHK(c, hipMemcpy(c->sor_relax, relax.data(), relax.size() * 8, hipMemcpyHostToDevice));
+        c->sor_nrelax = (int)relax.size();
+    }
+    return 0;
+}
+
+template <typename T>
+int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mode, double omega, double tol,
+                      double bnorm, int64_t max_sweeps, int nblk, int* host_done) {
+    const int batch = 32;
+    *host_done = 0;
+    int64_t l = 1;
+    while (l <= max_sweeps && !*host_done) {
+        const int64_t lend = std::min<int64_t>(max_sweeps, l + batch - 1);
+        for (; l <= lend; ++l) {
+            for (int color = 0; color < 2; ++color) {
+                swm::SorArgs<T> a{};
+                a.Nx = c->Nx; a.Ny = c->Ny; a.P = c->P; a.color = color; a.sweep = (int)l;
+                a.chi = (T*)dchi; a.B = (const T*)dB; a.mask = c->mask;
+                a.relax = ref_mode ? (const double*)c->sor_relax : nullptr;
+                a.nrelax = c->sor_nrelax; a.omega = omega;
+                a.c1t = (const T*)tabs[0]; a.c3t = (const T*)tabs[1]; a.c4t = (const T*)tabs[2]; a.c5t = (const T*)tabs[3];
+                a.partial = c->sor_partial + color * nblk;
+                a.done = c->sor_done;
+                a.residual_mode = ref_mode ? 0 : 1;
+                hipLaunchKernelGGL(swm::sor_sweep<T>, dim3(nblk), dim3(256), 0, c->stream, a);
+            }
+            hipLaunchKernelGGL(swm::sor_check, dim3(1), dim3(256), 0, c->stream, (const double*)c->sor_partial, 2 * nblk,
+                               (int)l, tol, bnorm, ref_mode ? 0 : 1, c->sor_done, c->sor_iters, c->sor_norm);
+        }
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(host_done, c->sor_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+int sor_common(swm_ctx* c, const double* B, double* chi, int ref_mode, double omega, double tol,
+                      int64_t max_sweeps, int fp32_sweeps, int64_t* iters, double* norm) {
+    const int Nx = c->Nx, Ny = c->Ny;
+    const int half = (Nx + 1) / 2;
+    const int nblk = (int)(((long)Ny * half + 255) / 256);
+    const bool f32 = fp32_sweeps != 0 || c->fp32;
+    int rc = sor_prepare(c, f32, nblk);
+    if (rc) return rc;
+    const int w = f32 ? 1 : 0;
+    void* dchi = c->sor_buf[w][0];
+    void* dB = c->sor_buf[w][1];
+    const size_t n = (size_t)Nx * Ny;
+    const size_t pitch_b = (size_t)c->P * (f32 ? 4 : 8);
+    if (f32) {
+        std::vector<float> fc(n), fb(n);
+        for (size_t k = 0; k < n; ++k) { fc[k] = (float)chi[k]; fb[k] = (float)B[k]; }
+        HIPCHK(c, hipMemcpy2D(dchi, pitch_b, fc.data(), Nx * 4, Nx * 4, Ny, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy2D(dB, pitch_b, fb.data(), Nx * 4, Nx * 4, Ny, hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(c, hipMemcpy2D(dchi, pitch_b, chi, Nx * 8, Nx * 8, Ny, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy2D(dB, pitch_b, B, Nx * 8, Nx * 8, Ny, hipMemcpyHostToDevice));
+    }
+    // ||B|| over ocean eta points, in FP64 whatever the sweep precision
+    double bnorm = 0.0;
+    if (!ref_mode) {
+        const int8_t* me = c->hmask[3].data();
+        for (size_t k = 0; k < n; ++k) {
+            const double b = f32 ? (double)(float)B[k] : B[k];
+            if (me[k]) bnorm += b * b;
+        }
+        bnorm = std::sqrt(bnorm);
+    }
+    HIPCHK(c, hipMemsetAsync(c->sor_done, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->sor_iters, 0, sizeof(long long), c->stream));
+    int host_done = 0;
+    rc = f32 ? sor_sweeps<float>(c, dchi, dB, c->sor_ftab[w], ref_mode, omega, tol, bnorm, max_sweeps, nblk, &host_done)
+             : sor_sweeps<double>(c, dchi, dB, c->sor_ftab[w], ref_mode, omega, tol, bnorm, max_sweeps, nblk, &host_done);
+    if (rc) return rc;
+    long long it = 0;
+    double nrm = 0.0;
+    HIPCHK(c, hipMemcpy(&it, c->sor_iters, sizeof(it), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(&nrm, c->sor_norm, sizeof(nrm), hipMemcpyDeviceToHost));
+    if (f32) {
+        std::vector<float> fc(n);
+        HIPCHK(c, hipMemcpy2D(fc.data(), Nx * 4, dchi, pitch_b, Nx * 4, Ny, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < n; ++k) chi[k] = fc[k];
+    } else {
+        HIPCHK(c, hipMemcpy2D(chi, Nx * 8, dchi, pitch_b, Nx * 8, Ny, hipMemcpyDeviceToHost));
+    }
+    *iters = it;
+    *norm = nrm;
+    return host_done ? 0 : 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int swm_gpu_sor(swm_ctx* c, const double* B, double* chi, double epsilon, int32_t first_guess, int64_t* iters,
+                double* anorm) {
+    if (!c || !B || !chi || !iters || !anorm) return 1;
+    const int mx = std::max(c->Nx, c->Ny);
+    const int64_t max_count = first_guess ? (int64_t)lrintf(1000.f * (float)mx) : (int64_t)lrintf(10.f * (float)mx);
+    return sor_common(c, B, chi, 1, 0.0, epsilon, max_count, 0, iters, anorm);
+}
+
+int swm_gpu_sor_fixed(swm_ctx* c, const double* B, double* chi, double omega, double rel_tol, int64_t max_sweeps,
+                      int32_t fp32_sweeps, int64_t* iters, double* rel_res) {
+    if (!c || !B || !chi || !iters || !rel_res) return 1;
+    return sor_common(c, B, chi, 0, omega, rel_tol, max_sweeps, fp32_sweeps, iters, rel_res);
+}
+
+int swm_gpu_divergence(swm_ctx* c, double* host_div) {
+    if (!c || !host_div) return 1;
+    const int n = c->Nx * c->Ny;
+    const int nb = (n + 255) / 256;
+    if (c->fp32)
+        hipLaunchKernelGGL(swm::divergence_kernel<float>, dim3(nb), dim3(256), 0, c->stream, c->Nx, c->Ny, c->P,
+                           (const float*)c->u[c->cur], (const float*)c->v[c->cur], c->mask, (const float*)c->dza,
+                           (const float*)c->tab[T_CV], (const float*)c->tab[T_CE], (float)(c->d.A * c->d.dTheta),
+                           (float)c->d.lbc, (float*)c->scratch);
+    else
+        hipLaunchKernelGGL(swm::divergence_kernel<double>, dim3(nb), dim3(256), 0, c->stream, c->Nx, c->Ny, c->P,
+                           (const double*)c->u[c->cur], (const double*)c->v[c->cur], c->mask, (const double*)c->dza,
+                           (const double*)c->tab[T_CV], (const double*)c->tab[T_CE], c->d.A * c->d.dTheta, c->d.lbc,
+                           (double*)c->scratch);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return download_grid(c, host_div, c->scratch);
+}
+
+const char* swm_gpu_last_error(const swm_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+void swm_gpu_destroy(swm_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+}  // extern "C"

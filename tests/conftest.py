@@ -1,0 +1,64 @@
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+def load_golden(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def linear_osc_setup(npz):
+    """Setup of test_fixed_length_output / test_oscfrc (LINEARISED_STATE_OF_REST,
+    BAROTROPIC, oscillating zonal forcing) from a golden fixture: the recorded
+    input fields and the experiment's numeric namelist values."""
+    from swm_amd import host
+    p = json.loads(str(npz["params"]))
+    Hin = npz["H"].astype(np.float64)
+    miss = (npz["H"] == npz["H_missing"]).astype(np.int8)
+    grids, dL, dT = host.init_domain(
+        int(p["domain_nl.nx"]), int(p["domain_nl.ny"]), float(p["domain_nl.lon_s"]), float(p["domain_nl.lon_e"]),
+        float(p["domain_nl.lat_s"]), float(p["domain_nl.lat_e"]), Hin, miss, float(p["domain_nl.h_overwrite"]),
+        0.0, 3, float(p["domain_nl.theta0"]), float(p["domain_nl.omega"]), float(p["domain_nl.a"]))
+    s = host.Setup(Nx=int(p["domain_nl.nx"]), Ny=int(p["domain_nl.ny"]), variant=host.LINEARISED_STATE_OF_REST,
+                   grids=grids, dLambda=dL, dTheta=dT, barotropic=True, Ah=float(p["model_nl.ah"]),
+                   dt=float(p["model_nl.dt"]), A=float(p["domain_nl.a"]), RHO0=float(p["domain_nl.rho0"]), lbc=0.0)
+    s.Nt = int(float(p["model_nl.run_length"]) / s.dt)
+    host.apply_linear_D_alias(s)
+    host.init_damping(s, [])
+    host.add_oscillation(s, host.COMP_X, npz["taux_re"].astype(np.float64), npz["taux_im"].astype(np.float64),
+                         float(p["swm_forcing_nl0.omega"]))
+    return host.finalize(s)
+
+
+def same(a, b):
+    """Bitwise-equal values (NaN positions must agree; +0 and -0 compare equal)."""
+    na, nb = np.isnan(a), np.isnan(b)
+    return bool(np.array_equal(na, nb) and np.array_equal(a[~na], b[~nb]))
+
+
+def gpu_available():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def gpu_lib():
+    if not gpu_available():
+        pytest.skip("no GPU")
+    from swm_amd import gpu
+    return gpu.load_library()

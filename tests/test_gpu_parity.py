@@ -1,0 +1,226 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+FP64 results must be bit-identical (NaN positions equal, +0/-0 equal): the
+kernels evaluate the reference expression trees in the same order without
+contraction.  FP32 storage is checked against the FP64 oracle with the stated
+1e-4 relative max-norm bar on eta.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import linear_osc_setup, load_golden, same
+from swm_amd import configs, gpu, host
+
+pytestmark = pytest.mark.gpu
+
+STATE = [("SWM_U", "u"), ("SWM_V", "v"), ("SWM_ETA", "eta"), ("G_U", "G_u"), ("G_V", "G_v"), ("G_ETA", "G_eta"),
+         ("G_U_M1", "G_u_m1"), ("G_V_M1", "G_v_m1"), ("G_ETA_M1", "G_eta_m1")]
+DERIVED = [("SWM_D", "D"), ("SWM_DH", "Dh"), ("SWM_DU", "Du"), ("SWM_DV", "Dv"), ("SWM_RELVORT", "zeta"),
+           ("SWM_POT", "Pot"), ("SWM_EDENS", "EDens"), ("SWM_MU", "MU"), ("SWM_MV", "MV"), ("F_X", "F_x"),
+           ("F_Y", "F_y"), ("F_ETA", "F_eta")]
+TRACER = [("CH", "CH"), ("G_CH", "G_CH"), ("G_CH_M1", "G_CH_m1"), ("C", "C"), ("uhc", "uhc"), ("vhc", "vhc")]
+
+
+def compare(g, o, names, ntrc=0, tag=""):
+    for gn, on in names:
+        a, b = g.get(gn), o.get(on)
+        assert same(a, b), f"{tag} {gn}: max |diff| {np.nanmax(np.abs(a - b))}"
+    for t in range(ntrc):
+        for gn, on in TRACER:
+            if tag == "init" and gn in ("uhc", "vhc"):
+                continue   # the reference fills these at the first TRC_advance
+            a, b = g.get(gn, t), o.get(on, t)
+            assert same(a, b), f"{tag} tracer {t} {gn}: max |diff| {np.nanmax(np.abs(a - b))}"
+
+
+def run_pair(s, checkpoints, names=STATE + DERIVED, ntrc=None):
+    ntrc = len(s.tracers) if ntrc is None else ntrc
+    g = gpu.SwmGpu(s).init_from_setup()
+    o = oracle.Oracle(s)
+    compare(g, o, names, ntrc, "init")
+    for n in checkpoints:
+        g.step(n - g.itt)
+        o.step(n - o.itt)
+        assert g.itt == o.itt == n
+        compare(g, o, names, ntrc, f"itt={n}")
+    g.close()
+
+
+def test_fully_nonlinear_mixing_tracer_islands(gpu_lib):
+    """C2 physics on a grid that is not a multiple of the 64x8 tile: AB ramp
+    (steps 1-3), islands, mixing, windstress, one tracer."""
+    s = configs.c2_nonlinear(seed=5, N=150, dt=10.0)
+    s2 = configs.c2_nonlinear(seed=5, N=150, dt=10.0)
+    assert s.Nx == 150 and s2.Nx == 150
+    run_pair(s, [1, 2, 3, 7, 40])
+
+
+def test_fully_nonlinear_rectangular_two_tracers(gpu_lib):
+    s = configs.basin(203, 77, 5e3, 4e3, variant=host.FULLY_NONLINEAR, H0=300.0, dt=8.0, Ah=150.0, f0=1e-4,
+                      beta=2e-11, drag=2e-6, lateral_mixing=True, land_fraction=0.1, seed=11)
+    host.add_windstress(s, host.COMP_X, configs.double_gyre_tau(s, 0.05))
+    host.add_windstress(s, host.COMP_Y, 0.3 * configs.double_gyre_tau(s, 0.05))
+    rng = np.random.default_rng(2)
+    host.add_tracer(s, configs.fourier_modes(s, rng), kappa_h=30.0, gamma_C=1e-6, C0=0.5, cons=1e-7)
+    host.add_tracer(s, configs.fourier_modes(s, rng, n_modes=4), kappa_h=5.0)
+    host.finalize(s)
+    s.eta0 = 0.2 * configs.fourier_modes(s, rng, n_modes=3)
+    run_pair(s, [1, 3, 25])
+
+
+def test_unstable_dt_nan_patterns_match(gpu_lib):
+    """At dt = 30 s the C2 configuration leaves AB3's stability region and
+    overflows; the non-finite pattern must still agree exactly."""
+    s = configs.c2_nonlinear(seed=1, N=128, dt=30.0)
+    run_pair(s, [30, 60], names=STATE)
+
+
+def test_quadratic_friction_sponges_full_arrays(gpu_lib):
+    """QUADRATIC_BOTTOM_FRICTION and spatially varying impl/gamma arrays
+    (stored as full device arrays, not row tables)."""
+    s = configs.basin(90, 70, 4e3, 4e3, variant=host.FULLY_NONLINEAR, H0=200.0, dt=6.0, Ah=80.0, f0=1e-4,
+                      beta=2e-11, lateral_mixing=True, quadratic_friction=True, gamma_sq=2.5e-3, land_fraction=0.05,
+                      seed=4)
+    rng = np.random.default_rng(9)
+    s.impl_u = s.impl_u * (1.0 + 1e-3 * rng.random(s.shape))
+    s.impl_eta = 1.0 + 1e-4 * rng.random(s.shape)
+    s.gamma_sq_v = s.gamma_sq_v * (1.0 + rng.random(s.shape))
+    host.add_windstress(s, host.COMP_X, configs.double_gyre_tau(s, 0.1) * (1 + 0.1 * rng.random(s.shape)))
+    host.finalize(s)
+    run_pair(s, [1, 2, 3, 30])
+
+
+def test_state_of_rest_fixture_and_reference_record(gpu_lib):
+    """LINEARISED_STATE_OF_REST + oscillating forcing (test_fixed_length_output):
+    the GPU equals the oracle bit for bit and the reference's recorded u to 1e-12."""
+    npz = load_golden("fixed_length_output.npz")
+    s = linear_osc_setup(npz)
+    g = gpu.SwmGpu(s).init_from_setup()
+    o = oracle.Oracle(s)
+    mv = float(npz["u_missing"])
+    for t, ref in zip(npz["time"], npz["u"]):
+        itt = int(round(t / s.dt))
+        g.step(itt - g.itt)
+        o.step(itt - o.itt)
+        compare(g, o, STATE + [("F_X", "F_x"), ("SWM_POT", "Pot"), ("SWM_EDENS", "EDens")], 0, f"itt={itt}")
+        u = g.get("SWM_U")
+        m = ref != mv
+        assert np.abs(u[m] - ref[m]).max() / np.abs(ref[m]).max() <= 1e-12
+    g.close()
+
+
+def test_c1_linear_small(gpu_lib):
+    s = configs.c1_linear(seed=2, N=72, dt=20.0)
+    run_pair(s, [1, 2, 3, 50, 200])
+
+
+def test_tiny_grid_periodic_wrap(gpu_lib):
+    """A grid narrower than one tile: every halo index wraps."""
+    s = configs.basin(7, 5, 5e3, 5e3, variant=host.FULLY_NONLINEAR, H0=100.0, dt=5.0, Ah=50.0, f0=1e-4,
+                      lateral_mixing=True, seed=1)
+    host.add_windstress(s, host.COMP_X, configs.double_gyre_tau(s, 0.1))
+    host.add_tracer(s, configs.fourier_modes(s, np.random.default_rng(0), n_modes=2), kappa_h=10.0)
+    host.finalize(s)
+    run_pair(s, [1, 5, 20])
+
+
+def test_resume_from_synced_state(gpu_lib):
+    """Checkpoint = (u, v, eta, 3 G slots, itt, CH, C, 3 G_CH slots): a context
+    resumed from a sync continues bit for bit."""
+    s = configs.c2_nonlinear(seed=7, N=80, dt=10.0)
+    a = gpu.SwmGpu(s).init_from_setup()
+    a.step(10)
+    st = {k: a.get(k) for k in ("SWM_U", "SWM_V", "SWM_ETA", "G_U_M1", "G_U", "G_V_M1", "G_V", "G_ETA_M1", "G_ETA")}
+    G = lambda f: np.stack([st[f"G_{f}_M1"], st[f"G_{f}"], st[f"G_{f}"]])
+    ch, c_, gch1, gch2 = a.get("CH", 0), a.get("C", 0), a.get("G_CH_M1", 0), a.get("G_CH", 0)
+    b = gpu.SwmGpu(s)
+    b.upload_state(st["SWM_U"], st["SWM_V"], st["SWM_ETA"], G("U"), G("V"), G("ETA"), itt=10)
+    t = s.tracers[0]
+    b.add_tracer(t["kappa_h"], ch, c_, G_CH=np.stack([gch1, gch2, gch2]), C0=t["C0"], gamma_C=t["gamma_C"],
+                 impl=1.0 + s.dt * t["cons"])
+    a.step(15)
+    b.step(15)
+    for k in ("SWM_U", "SWM_V", "SWM_ETA", "G_U", "G_ETA"):
+        assert same(a.get(k), b.get(k)), k
+    assert same(a.get("CH", 0), b.get("CH", 0))
+    a.close()
+    b.close()
+
+
+def test_full_size_c2_steps(gpu_lib):
+    """The benchmark shape (1024 x 1024, FULLY_NONLINEAR + mixing + tracer,
+    5% islands) for a few steps, bit-identical to the oracle."""
+    s = configs.c2_nonlinear(seed=0, N=1024, dt=10.0)
+    run_pair(s, [3, 6], names=STATE + [("SWM_POT", "Pot"), ("SWM_EDENS", "EDens")])
+
+
+def test_fp32_storage_error_bar(gpu_lib):
+    """FP32 fields and arithmetic stay within 1e-4 relative max-norm of the FP64
+    oracle on eta (stable dt, 400 steps)."""
+    s = configs.c2_nonlinear(seed=3, N=128, dt=10.0, tracer=False)
+    g = gpu.SwmGpu(s, precision=gpu.FP32).init_from_setup()
+    o = oracle.Oracle(s)
+    g.step(400)
+    o.step(400)
+    oc = s.grids["eta"].ocean == 1
+    e32, e64 = g.get("SWM_ETA")[oc], o.get("eta")[oc]
+    rel = np.abs(e32 - e64).max() / np.abs(e64).max()
+    assert rel <= 1e-4, rel
+    g.close()
+
+
+def test_divergence_matches(gpu_lib):
+    s = configs.c2_nonlinear(seed=2, N=100, dt=10.0, tracer=False)
+    g = gpu.SwmGpu(s).init_from_setup()
+    o = oracle.Oracle(s)
+    g.step(20)
+    o.step(20)
+    assert same(g.divergence(), o.divergence(o.get("u"), o.get("v")))
+    g.close()
+
+
+def test_sor_reference_mode_bit_exact(gpu_lib):
+    """main_ElSolv_SOR with the reference EPS = 1e-4 (calc_lib.h:12): same sweep
+    count and bit-identical chi (red-black updates are order independent; only
+    the anorm summation order differs).  Also the non-converged exit after
+    max_count = 10*max(Nx,Ny) sweeps when there is no first guess."""
+    s = configs.basin(64, 48, 10e3, 10e3, variant=host.FULLY_NONLINEAR, H0=500.0, dt=10.0, seed=3,
+                      land_fraction=0.05)
+    host.finalize(s)
+    rng = np.random.default_rng(1)
+    o = oracle.Oracle(s)
+    u = rng.standard_normal(s.shape) * s.grids["u"].ocean
+    v = rng.standard_normal(s.shape) * s.grids["v"].ocean
+    B = o.divergence(u, v)
+    g = gpu.SwmGpu(s)
+    for first_guess, converged in ((True, True), (False, False)):
+        chi_o, it_o, an_o, rc_o = o.sor(B, np.zeros(s.shape), epsilon=1e-4, first_guess=first_guess)
+        chi_g, it_g, an_g, ok = g.sor(B, np.zeros(s.shape), epsilon=1e-4, first_guess=first_guess)
+        assert ok == converged == (rc_o == 0)
+        assert it_g == it_o
+        assert same(chi_g, chi_o)
+        assert abs(an_g - an_o) <= 1e-12 * an_o
+    g.close()
+
+
+def test_sor_fixed_omega_fp64_and_fp32_sweeps(gpu_lib):
+    s = configs.basin(96, 64, 10e3, 10e3, variant=host.FULLY_NONLINEAR, H0=500.0, dt=10.0, seed=5)
+    host.finalize(s)
+    rng = np.random.default_rng(4)
+    o = oracle.Oracle(s)
+    B = o.divergence(rng.standard_normal(s.shape) * s.grids["u"].ocean, rng.standard_normal(s.shape) * s.grids["v"].ocean)
+    chi_o, it_o, rr_o, rc_o = o.sor_fixed(B, np.zeros(s.shape), 1.9, 1e-10, 5000)
+    g = gpu.SwmGpu(s)
+    chi_g, it_g, rr_g, ok = g.sor_fixed(B, np.zeros(s.shape), 1.9, 1e-10, 5000)
+    assert ok and rc_o == 0
+    assert abs(it_g - it_o) <= 1
+    if it_g == it_o:
+        assert same(chi_g, chi_o)
+    chi_f, it_f, rr_f, ok_f = g.sor_fixed(B, np.zeros(s.shape), 1.9, 1e-6, 5000, fp32_sweeps=True)
+    assert ok_f
+    oc = s.grids["eta"].ocean == 1
+    ref = chi_o[oc] - chi_o[oc].mean()
+    got = chi_f[oc] - chi_f[oc].mean()
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-3
+    g.close()
