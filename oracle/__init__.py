@@ -79,6 +79,7 @@ def lib():
                                     C.POINTER(C.c_int64), _dp]
         L.orc_sor_fixed.restype = C.c_int
         L.orc_divergence.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.orc_sor_apply.argtypes = [C.c_void_p, _dp, _dp]
         _lib = L
     return _lib
 
@@ -171,6 +172,12 @@ class Oracle:
         v = np.ascontiguousarray(v, np.float64)
         out = np.empty(self.s.shape)
         lib().orc_divergence(self._h, u.ctypes.data_as(_dp), v.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+        return out
+
+    def sor_apply(self, chi):
+        chi = np.ascontiguousarray(chi, np.float64)
+        out = np.empty(self.s.shape)
+        lib().orc_sor_apply(self._h, chi.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
         return out
 
     def sor(self, B, chi, epsilon=1e-4, first_guess=True):

@@ -783,9 +783,9 @@ int orc_get(const orc_model *m, const char *name, int tracer, double *out) {
     else if (!strcmp(name, "G_u")) src = m->G_u[NG - 1];
     else if (!strcmp(name, "G_v")) src = m->G_v[NG - 1];
     else if (!strcmp(name, "G_eta")) src = m->G_eta[NG - 1];
-    else if (!strcmp(name, "G_u_m1")) src = m->G_u[NG - 2];
-    else if (!strcmp(name, "G_v_m1")) src = m->G_v[NG - 2];
-    else if (!strcmp(name, "G_eta_m1")) src = m->G_eta[NG - 2];
+    else if (!strcmp(name, "G_u_m1")) src = m->G_u[0];      /* oldest slot G(1) */
+    else if (!strcmp(name, "G_v_m1")) src = m->G_v[0];
+    else if (!strcmp(name, "G_eta_m1")) src = m->G_eta[0];
     else if (!strcmp(name, "D")) src = m->D;
     else if (!strcmp(name, "Dh")) src = m->Dh;
     else if (!strcmp(name, "Du")) src = m->Du;
@@ -807,7 +807,7 @@ int orc_get(const orc_model *m, const char *name, int tracer, double *out) {
     else if (t && !strcmp(name, "CH")) src = t->CH[0];
     else if (t && !strcmp(name, "C")) src = t->C;
     else if (t && !strcmp(name, "G_CH")) src = t->G_CH[TRC_NG - 1];
-    else if (t && !strcmp(name, "G_CH_m1")) src = t->G_CH[TRC_NG - 2];
+    else if (t && !strcmp(name, "G_CH_m1")) src = t->G_CH[0];
     else if (t && !strcmp(name, "uhc")) src = t->uhc;
     else if (t && !strcmp(name, "vhc")) src = t->vhc;
     if (!src) return 1;
@@ -886,7 +886,9 @@ int orc_sor(orc_model *m, const double *B, double *chi, double epsilon, int firs
         anorm = sqrt(anorm);
         if (anorm < epsilon) { *iters = l; *anorm_out = anorm; return 0; }
     }
-    *iters = max_count; *anorm_out = sqrt(anorm);
+    /* anorm already holds sqrt(sum res^2) of the last sweep (the reference's
+     * warning line prints a further sqrt of it, ElSolv_SOR.f90:220) */
+    *iters = max_count; *anorm_out = anorm;
     return 1;
 }
 
@@ -920,6 +922,15 @@ int orc_sor_fixed(orc_model *m, const double *B, double *chi, double omega, doub
     }
     *iters = max_sweeps; *rel_res = bnorm > 0.0 ? sqrt(rn) / bnorm : sqrt(rn);
     return 1;
+}
+
+/* The SOR stencil itself, (L chi)(i,j) on ocean eta points, 0 elsewhere. */
+void orc_sor_apply(orc_model *m, const double *chi, double *out) {
+    sor_init(m);
+    const int8_t *oe = m->ocean[ORC_GRID_ETA];
+    for (int j = 0; j < m->Ny; ++j)
+        for (int i = 0; i < m->Nx; ++i)
+            out[IX(m, i, j)] = oe[IX(m, i, j)] ? sor_lhs(m, chi, i, j) : 0.0;
 }
 
 void orc_destroy(orc_model *m) {

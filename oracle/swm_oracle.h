@@ -75,6 +75,8 @@ int orc_sor(orc_model *m, const double *B, double *chi, double epsilon, int firs
 /* fixed-omega mode with relative-residual stopping rule */
 int orc_sor_fixed(orc_model *m, const double *B, double *chi, double omega, double rel_tol,
                   int64_t max_sweeps, int64_t *iters, double *rel_res);
+/* apply the SOR operator (L chi) on ocean eta points */
+void orc_sor_apply(orc_model *m, const double *chi, double *out);
 /* computeDivergence(u, v, div, u_grid, v_grid) */
 void orc_divergence(const orc_model *m, const double *u, const double *v, double *div);
 #endif

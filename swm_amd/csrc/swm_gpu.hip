@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -23,8 +24,11 @@ using swm::StepArgs;
 
 namespace {
 
-constexpr int kTX = 64;   // step-kernel tile (outputs per block): 64 x 8
-constexpr int kTY = 8;
+// Step-kernel tile shapes (outputs per 256-thread block).  The default is the
+// shape measured fastest on MI355X; the others stay selectable for tuning.
+struct TileShape { int tx, ty; };
+constexpr TileShape kTiles[] = {{64, 8}, {32, 8}, {64, 4}, {128, 4}, {32, 16}};
+constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 template <typename T>
 struct FieldH {              // host description of an uploaded static field
@@ -63,6 +67,9 @@ struct swm_ctx {
     void* Gv[3] = {};
     void* Ge[3] = {};
     int ring = 0;                // slot holding G(1) (oldest); G(2) = ring+1, newest = ring+2
+    int tile = 0;                // index into kTiles
+    bool use_graph = true;
+    hipGraphExec_t graphs[6] = {};   // steady-state 6-step cycles, keyed by (cur, ring)
     int64_t itt = 0;
     uint8_t* mask = nullptr;
     std::vector<int8_t> hmask[4];
@@ -305,8 +312,9 @@ StepArgs<T> make_args(swm_ctx* c) {
     const swm_desc& d = c->d;
     StepArgs<T> a{};
     a.Nx = c->Nx; a.Ny = c->Ny; a.P = c->P;
-    a.ntx = (c->Nx + kTX - 1) / kTX;
-    a.nty = (c->Ny + kTY - 1) / kTY;
+    const TileShape ts = kTiles[c->tile];
+    a.ntx = (c->Nx + ts.tx - 1) / ts.tx;
+    a.nty = (c->Ny + ts.ty - 1) / ts.ty;
     a.nblk = a.ntx * a.nty;
     a.mix = (d.flags & SWM_LATERAL_MIXING) ? 1 : 0;
     a.quad = (d.flags & SWM_QUADRATIC_BOTTOM_FRICTION) ? 1 : 0;
@@ -364,13 +372,13 @@ StepArgs<T> make_args(swm_ctx* c) {
     return a;
 }
 
-template <typename T, bool DIAG>
-int launch_step_t(swm_ctx* c, const StepArgs<T>& a) {
+template <typename T, int TX, int TY, bool DIAG>
+void launch_tile(swm_ctx* c, const StepArgs<T>& a) {
     const int var = c->d.variant;
     const int nt = (int)c->trc.size();
     dim3 grid(a.nblk), block(256);
-#define SWM_LAUNCH(V, N)                                                                                   \
-    hipLaunchKernelGGL((swm::step_kernel<T, V, N, kTX, kTY, DIAG>), grid, block, 0, c->stream, a)
+#define SWM_LAUNCH(V, N) \
+    hipLaunchKernelGGL((swm::step_kernel<T, V, N, TX, TY, DIAG>), grid, block, 0, c->stream, a)
     if (var == SWM_FULLY_NONLINEAR) {
         if (nt == 0) SWM_LAUNCH(swm::VAR_FN, 0);
         else if (nt == 1) SWM_LAUNCH(swm::VAR_FN, 1);
@@ -381,6 +389,21 @@ int launch_step_t(swm_ctx* c, const StepArgs<T>& a) {
         else SWM_LAUNCH(swm::VAR_SOR, 2);
     }
 #undef SWM_LAUNCH
+}
+
+template <typename T, bool DIAG>
+int launch_step_t(swm_ctx* c, const StepArgs<T>& a) {
+    if (DIAG) {
+        launch_tile<T, 64, 8, true>(c, a);
+    } else {
+        switch (c->tile) {
+        case 1: launch_tile<T, 32, 8, false>(c, a); break;
+        case 2: launch_tile<T, 64, 4, false>(c, a); break;
+        case 3: launch_tile<T, 128, 4, false>(c, a); break;
+        case 4: launch_tile<T, 32, 16, false>(c, a); break;
+        default: launch_tile<T, 64, 8, false>(c, a); break;
+        }
+    }
     HIPCHK(c, hipGetLastError());
     return 0;
 }
@@ -413,6 +436,49 @@ int step_one(swm_ctx* c) {
     return 0;
 }
 
+void drop_graphs(swm_ctx* c) {
+    for (auto& g : c->graphs)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
+// Steady state (itt >= 3, no first-step tracer buffer, no oscillating stream):
+// the launch arguments repeat with the period 6 of the ping-pong x ring roles,
+// so six steps are captured once per phase and replayed as one graph.
+template <typename T>
+int step_many(swm_ctx* c, int64_t nsteps) {
+    const bool has_osc = c->d.osc_re[0] || c->d.osc_re[1] || c->d.osc_re[2];
+    int64_t done = 0;
+    while (done < nsteps) {
+        bool cbuf = false;
+        for (auto& tr : c->trc) cbuf = cbuf || tr.cbuf_valid;
+        const bool steady = c->use_graph && !has_osc && !cbuf && c->itt >= 2;
+        if (steady && nsteps - done >= 6) {
+            const int key = c->cur * 3 + c->ring;
+            if (!c->graphs[key]) {
+                hipGraph_t g = nullptr;
+                HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                int rc = 0;
+                for (int k = 0; k < 6 && !rc; ++k) rc = step_one<T>(c);
+                hipError_t e = hipStreamEndCapture(c->stream, &g);
+                if (rc) return rc;
+                if (e != hipSuccess) return fail(c, std::string("graph capture: ") + hipGetErrorString(e), 3);
+                e = hipGraphInstantiate(&c->graphs[key], g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e != hipSuccess) return fail(c, std::string("graph instantiate: ") + hipGetErrorString(e), 3);
+                c->itt -= 6;   // capture recorded the six steps without running them
+            }
+            HIPCHK(c, hipGraphLaunch(c->graphs[key], c->stream));
+            c->itt += 6;
+            done += 6;
+        } else {
+            int rc = step_one<T>(c);
+            if (rc) return rc;
+            done += 1;
+        }
+    }
+    return 0;
+}
+
 template <typename T>
 int materialise(swm_ctx* c) {
     for (int k = 0; k < 14; ++k)
@@ -420,7 +486,10 @@ int materialise(swm_ctx* c) {
     for (size_t n = 0; n < c->trc.size(); ++n)
         for (int k = 0; k < 3; ++k)
             if (!c->tdiag[n][k] && !(c->tdiag[n][k] = alloc_grid(c))) return fail(c, "out of device memory", 3);
+    const int saved_tile = c->tile;
+    c->tile = 0;   // the materialisation kernel uses the 64 x 8 tile
     StepArgs<T> a = make_args<T>(c);
+    c->tile = saved_tile;
     T** outs[14] = {&a.dD, &a.dDh, &a.dDu, &a.dDv, &a.dzeta, &a.dPot, &a.dEd, &a.dMU, &a.dMV,
                     &a.dFx, &a.dFy, &a.dFe, &a.dLu, &a.dLv};
     for (int k = 0; k < 14; ++k) *outs[k] = (T*)c->diag[k];
@@ -475,6 +544,8 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
     c->Ny = d->Ny;
     c->fp32 = d->precision == SWM_FP32;
     c->esz = c->fp32 ? 4 : 8;
+    if (const char* e = getenv("SWM_GPU_TILE")) c->tile = std::max(0, std::min(kNumTiles - 1, atoi(e)));
+    if (const char* e = getenv("SWM_GPU_GRAPH")) c->use_graph = atoi(e) != 0;
     const int align = c->fp32 ? 32 : 16;   // 128-byte aligned rows
     c->P = (c->Nx + align - 1) / align * align;
     for (int g = 0; g < 4; ++g) {
@@ -553,6 +624,7 @@ int swm_gpu_upload_state(swm_ctx* c, const double* u, const double* v, const dou
     if (!u || !v || !eta) return fail(c, "u, v and eta are required", 1);
     if (itt < 0) return fail(c, "itt must be >= 0", 1);
     const size_t n = (size_t)c->Nx * c->Ny;
+    drop_graphs(c);
     c->cur = 0;
     c->ring = 0;
     int rc;
@@ -575,6 +647,7 @@ int swm_gpu_add_tracer(swm_ctx* c, const swm_tracer_desc* t, int32_t* tracer_id)
     if (c->d.variant != SWM_FULLY_NONLINEAR)
         return fail(c, "tracers need FULLY_NONLINEAR (h = D); the linear D alias makes C = CH/0 on land", 4);
     const size_t n = (size_t)c->Nx * c->Ny;
+    drop_graphs(c);
     Tracer tr;
     tr.kappa = t->kappa_h;
     int rc;
@@ -619,10 +692,8 @@ int swm_gpu_step(swm_ctx* c, int64_t nsteps) {
     if (!c) return 1;
     if (nsteps < 0) return fail(c, "nsteps must be >= 0", 1);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    for (int64_t s = 0; s < nsteps; ++s) {
-        int rc = c->fp32 ? step_one<float>(c) : step_one<double>(c);
-        if (rc) return rc;
-    }
+    int rc = c->fp32 ? step_many<float>(c, nsteps) : step_many<double>(c, nsteps);
+    if (rc) return rc;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0.f;
@@ -867,6 +938,7 @@ const char* swm_gpu_last_error(const swm_ctx* c) { return c ? c->err.c_str() : g
 void swm_gpu_destroy(swm_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);

@@ -1,0 +1,156 @@
+!> ISO_C_BINDING interface of the MI355X SWM time-stepping library.
+!!
+!! A build of the reference model uses this module in place of the CPU bodies
+!! of SWM_timestep / SWM_advance (src/swm_module.f90:71-139), TRC_timestep /
+!! TRC_advance (src/tracer_module.f90:109-146) and main_ElSolv_SOR
+!! (src/ElSolv_SOR.f90:150-222): model.f90 keeps its call order, fields stay on
+!! the device, and a diag hook copies registered fields back when a task is due.
+!! The C declarations are in include/swm_gpu.h.
+module swm_gpu_shim
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+
+  integer(c_int32_t), parameter, public :: SWM_GPU_ABI_VERSION = 1
+  integer(c_int32_t), parameter, public :: SWM_FULLY_NONLINEAR = 1, SWM_LINEARISED_STATE_OF_REST = 3
+  integer(c_int32_t), parameter, public :: SWM_LATERAL_MIXING = 1, SWM_QUADRATIC_BOTTOM_FRICTION = 2, &
+                                           SWM_BAROTROPIC = 4, SWM_TAU_SCALE = 8
+  integer(c_int32_t), parameter, public :: SWM_FP64 = 0, SWM_FP32 = 1
+
+  !> swm_grid_desc
+  type, bind(C), public :: swm_grid_desc
+    type(c_ptr) :: ocean = c_null_ptr
+    type(c_ptr) :: H = c_null_ptr
+    type(c_ptr) :: cos_lat = c_null_ptr
+    type(c_ptr) :: tan_lat = c_null_ptr
+    type(c_ptr) :: f = c_null_ptr
+  end type swm_grid_desc
+
+  !> swm_desc
+  type, bind(C), public :: swm_desc
+    integer(c_int32_t) :: abi_version = SWM_GPU_ABI_VERSION
+    integer(c_int32_t) :: Nx = 0, Ny = 0, variant = 0, flags = 0, precision = SWM_FP64
+    real(c_double) :: G = 0, Ah = 0, dt = 0, A = 0, dLambda = 0, dTheta = 0, RHO0 = 0, minD = 1, &
+                      lbc = 0, ab_chi = 0.1_c_double, tau_scale = 0
+    type(swm_grid_desc) :: grid(4)
+    type(c_ptr) :: impl_u = c_null_ptr, impl_v = c_null_ptr, impl_eta = c_null_ptr
+    type(c_ptr) :: gamma_sq_u = c_null_ptr, gamma_sq_v = c_null_ptr
+    type(c_ptr) :: F_x_const = c_null_ptr, F_y_const = c_null_ptr, F_eta_const = c_null_ptr
+    type(c_ptr) :: tau_x = c_null_ptr, tau_y = c_null_ptr
+    type(c_ptr) :: osc_re(3) = c_null_ptr, osc_im(3) = c_null_ptr
+    real(c_double) :: osc_omega(3) = 0
+  end type swm_desc
+
+  !> swm_tracer_desc
+  type, bind(C), public :: swm_tracer_desc
+    real(c_double) :: kappa_h = 0
+    type(c_ptr) :: CH = c_null_ptr, C = c_null_ptr, G_CH = c_null_ptr
+    type(c_ptr) :: C0 = c_null_ptr, gamma_C = c_null_ptr, impl = c_null_ptr
+  end type swm_tracer_desc
+
+  interface
+    integer(c_int) function swm_gpu_create(d, ctx) bind(C, name="swm_gpu_create")
+      import :: c_int, c_ptr, swm_desc
+      type(swm_desc), intent(in) :: d
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function swm_gpu_upload_state(ctx, u, v, eta, G_u, G_v, G_eta, itt) &
+        bind(C, name="swm_gpu_upload_state")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, u, v, eta, G_u, G_v, G_eta
+      integer(c_int64_t), value :: itt
+    end function
+    integer(c_int) function swm_gpu_add_tracer(ctx, t, id) bind(C, name="swm_gpu_add_tracer")
+      import :: c_int, c_ptr, c_int32_t, swm_tracer_desc
+      type(c_ptr), value :: ctx
+      type(swm_tracer_desc), intent(in) :: t
+      integer(c_int32_t), intent(out) :: id
+    end function
+    integer(c_int) function swm_gpu_step(ctx, nsteps) bind(C, name="swm_gpu_step")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nsteps
+    end function
+    integer(c_int64_t) function swm_gpu_itt(ctx) bind(C, name="swm_gpu_itt")
+      import :: c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function swm_gpu_sync(ctx, name, tracer_id, out) bind(C, name="swm_gpu_sync")
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: name
+      integer(c_int32_t), value :: tracer_id
+      type(c_ptr), value :: out
+    end function
+    integer(c_int) function swm_gpu_sor(ctx, B, chi, epsilon, first_guess, iters, anorm) &
+        bind(C, name="swm_gpu_sor")
+      import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx, B, chi
+      real(c_double), value :: epsilon
+      integer(c_int32_t), value :: first_guess
+      integer(c_int64_t), intent(out) :: iters
+      real(c_double), intent(out) :: anorm
+    end function
+    type(c_ptr) function swm_gpu_last_error(ctx) bind(C, name="swm_gpu_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    subroutine swm_gpu_destroy(ctx) bind(C, name="swm_gpu_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+  end interface
+
+  public :: swm_gpu_create, swm_gpu_upload_state, swm_gpu_add_tracer, swm_gpu_step, swm_gpu_itt, &
+            swm_gpu_sync, swm_gpu_sor, swm_gpu_last_error, swm_gpu_destroy
+  public :: gpu_check, gpu_sync_field, gpu_main_ElSolv_SOR
+
+contains
+
+  !> Turn a non-zero status into the reference's fatal error path (log_fatal -> STOP 1).
+  subroutine gpu_check(ctx, rc, what)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (rc == 0) return
+    call c_f_pointer(swm_gpu_last_error(ctx), msg, [512])
+    n = 0
+    do while (n < 512)
+      if (msg(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    write (*, '(A,": status ",I0,": ",512A)') what, rc, msg(1:n)
+    stop 1
+  end subroutine gpu_check
+
+  !> Register read of a due diag task: copy device field `name` into `field`.
+  subroutine gpu_sync_field(ctx, name, tracer_id, field)
+    type(c_ptr), intent(in) :: ctx
+    character(*), intent(in) :: name
+    integer, intent(in) :: tracer_id
+    real(c_double), dimension(:, :), target, contiguous, intent(inout) :: field
+    call gpu_check(ctx, swm_gpu_sync(ctx, trim(name)//c_null_char, int(tracer_id, c_int32_t), c_loc(field)), &
+                   "swm_gpu_sync "//trim(name))
+  end subroutine gpu_sync_field
+
+  !> main_ElSolv_SOR(B, chi, epsilon, first_guess) with the reference's contract.
+  subroutine gpu_main_ElSolv_SOR(ctx, B, chi, epsilon, first_guess)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), dimension(:, :), target, contiguous, intent(in) :: B
+    real(c_double), dimension(:, :), target, contiguous, intent(inout) :: chi
+    real(c_double), intent(in) :: epsilon
+    logical, intent(in) :: first_guess
+    integer(c_int64_t) :: iters
+    real(c_double) :: anorm
+    integer(c_int) :: rc
+    rc = swm_gpu_sor(ctx, c_loc(B), c_loc(chi), epsilon, merge(1_c_int32_t, 0_c_int32_t, first_guess), iters, anorm)
+    if (rc == 1) then
+      write (*, '(A,X,I6,X,A,X,E11.4)') "Elliptic solver did not converge after", iters, "iterations. Residual:", anorm
+    else
+      call gpu_check(ctx, rc, "swm_gpu_sor")
+    end if
+  end subroutine gpu_main_ElSolv_SOR
+
+end module swm_gpu_shim

@@ -362,8 +362,9 @@ def add_windstress(s: Setup, comp: int, tau: np.ndarray, constant_in_time: bool 
     Otherwise a constant stream is folded once into F_*_const with the
     (overwritten) H_u/H_v, as SWM_forcing_getForcing(isTDF=.FALSE.) does.
     """
-    if s.F_x_const is None:
-        s.F_x_const, s.F_y_const, s.F_eta_const = np.zeros(s.shape), np.zeros(s.shape), np.zeros(s.shape)
+    for name in ("F_x_const", "F_y_const", "F_eta_const"):
+        if getattr(s, name) is None:
+            setattr(s, name, np.zeros(s.shape))
     tau = np.asarray(tau, dtype=np.float64)
     if s.variant == FULLY_NONLINEAR or not constant_in_time:
         if not constant_in_time and s.variant != FULLY_NONLINEAR:
@@ -380,16 +381,18 @@ def add_windstress(s: Setup, comp: int, tau: np.ndarray, constant_in_time: bool 
 
 def add_oscillation(s: Setup, comp: int, re: np.ndarray, im: np.ndarray, omega: float) -> None:
     """An oscillating forcing stream (processOscillation, 326-374)."""
-    if s.F_x_const is None:
-        s.F_x_const, s.F_y_const, s.F_eta_const = np.zeros(s.shape), np.zeros(s.shape), np.zeros(s.shape)
+    for name in ("F_x_const", "F_y_const", "F_eta_const"):
+        if getattr(s, name) is None:
+            setattr(s, name, np.zeros(s.shape))
     s.osc.append((comp, np.asarray(re, dtype=np.float64), np.asarray(im, dtype=np.float64), float(omega)))
 
 
 def finalize(s: Setup) -> Setup:
     """Fill defaults for anything the configuration did not set."""
     z = lambda: np.zeros(s.shape)
-    if s.F_x_const is None:
-        s.F_x_const, s.F_y_const, s.F_eta_const = z(), z(), z()
+    for name in ("F_x_const", "F_y_const", "F_eta_const"):
+        if getattr(s, name) is None:
+            setattr(s, name, z())
     if s.impl_u is None:
         init_damping(s, [])
     if s.quadratic_friction and s.gamma_sq_u is None:
