@@ -4,7 +4,8 @@
 // values per row), builds per-row coefficient tables with the reference's
 // expression order in FP64 on the host, keeps u/v/eta (ping-pong) and the AB
 // history (3-slot ring: the reference's G shift becomes a role rotation) on the
-// device, and launches one fused step kernel per time step.
+// device in one allocation of planes, and launches one fused step kernel per
+// time step (steady-state cycles replayed as HIP graphs).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,36 +15,43 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "swm_gpu.h"
 #include "swm_kernels.h"
 
+using swm::Dyn;
 using swm::Fld;
-using swm::StepArgs;
+using swm::Statics;
 
 namespace {
 
-// Step-kernel tile shapes (outputs per 256-thread block).  The default is the
-// shape measured fastest on MI355X; the others stay selectable for tuning.
+// Step-kernel tile shapes (outputs per 256-thread block).  Index 0 is the
+// default; the others stay selectable (SWM_GPU_TILE) for tuning.
 struct TileShape { int tx, ty; };
 constexpr TileShape kTiles[] = {{64, 8}, {32, 8}, {64, 4}, {128, 4}, {32, 16}};
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+constexpr int kMaxTracers = 2;
 
-template <typename T>
-struct FieldH {              // host description of an uploaded static field
-    T* full = nullptr;
-    T* rows = nullptr;
+enum FieldId {
+    F_HE, F_HU, F_HV, F_IMU, F_IMV, F_IME, F_GSU, F_GSV, F_FXC, F_FYC, F_FEC, F_TAUX, F_TAUY,
+    F_ORX, F_OIX, F_ORY, F_OIY, F_ORE, F_OIE, F_COUNT
+};
+enum TabId {
+    T_CU, T_CV, T_CE, T_CH, T_FH, T_ZA, T_EDU, T_ACE, T_LMU, T_LMV, T_PLLC, T_PLTC, T_TA, T_TB, T_TC,
+    T_T9, T_T11, T_T12, T_COUNT
+};
+
+struct FieldDev {            // device pointers of an uploaded static field
+    void* full = nullptr;
+    void* rows = nullptr;
 };
 
 struct Tracer {
     double kappa = 0;
-    void* CH[2] = {nullptr, nullptr};
-    void* G[3] = {nullptr, nullptr, nullptr};
-    void* Cbuf = nullptr;
     bool cbuf_valid = false;
-    FieldH<double> C0d, gCd, impld;   // FP64 pointers
-    FieldH<float> C0f, gCf, implf;    // FP32 pointers
+    FieldDev C0, gC, impl;
 };
 
 }  // namespace
@@ -58,30 +66,27 @@ struct swm_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_total_ms = 0, last_kernel_ms = 0;
     std::vector<void*> allocs;
-    // state
-    void* u[2] = {};
-    void* v[2] = {};
-    void* eta[2] = {};
+    bool has_osc = false, has_taux = false, has_tauy = false;
+    // state planes (swm::PL_*), one allocation
+    void* state = nullptr;
+    long long plane = 0;         // elements per plane
     int cur = 0;                 // ping-pong index of level N0
-    void* Gu[3] = {};
-    void* Gv[3] = {};
-    void* Ge[3] = {};
     int ring = 0;                // slot holding G(1) (oldest); G(2) = ring+1, newest = ring+2
-    int tile = 0;                // index into kTiles
-    bool use_graph = true;
-    hipGraphExec_t graphs[6] = {};   // steady-state 6-step cycles, keyed by (cur, ring)
     int64_t itt = 0;
     uint8_t* mask = nullptr;
     std::vector<int8_t> hmask[4];
-    // static fields and tables (device pointers, element type per precision)
-    void* fields[19][2] = {};    // [field][full|rows]
-    void* tab[32] = {};
+    FieldDev fields[F_COUNT];
+    void* tab[T_COUNT] = {};
+    void* statics = nullptr;     // device Statics<T>
     double ab[5][5] = {};
     std::vector<Tracer> trc;
+    int tile = 0;
+    bool use_graph = true;
+    hipGraphExec_t graphs[6] = {};   // steady-state 6-step cycles, keyed by (cur, ring)
     // diagnostics scratch
     void* diag[14] = {};
-    void* tdiag[2][3] = {};
-    void* scratch = nullptr;     // Ny*P elements (sync staging, divergence)
+    void* tdiag[kMaxTracers][3] = {};
+    void* scratch = nullptr;     // one plane (divergence output)
     // SOR
     void* sor_tab[4] = {};
     void* sor_relax = nullptr;
@@ -96,15 +101,6 @@ struct swm_ctx {
 };
 
 namespace {
-
-enum FieldId {
-    F_HE, F_HU, F_HV, F_IMU, F_IMV, F_IME, F_GSU, F_GSV, F_FXC, F_FYC, F_FEC, F_TAUX, F_TAUY,
-    F_ORX, F_OIX, F_ORY, F_OIY, F_ORE, F_OIE, F_COUNT
-};
-enum TabId {
-    T_CU, T_CV, T_CE, T_CH, T_FH, T_ZA, T_EDU, T_ACE, T_LMU, T_LMV, T_PLLC, T_PLTC, T_TA, T_TB, T_TC,
-    T_T9, T_T11, T_T12, T_COUNT
-};
 
 int fail(swm_ctx* c, const std::string& msg, int code = 2) {
     if (c) c->err = msg;
@@ -125,6 +121,13 @@ void* dalloc(swm_ctx* c, size_t bytes) {
     return p;
 }
 
+void dfree(swm_ctx* c, void* p) {
+    if (!p) return;
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+    if (it != c->allocs.end()) c->allocs.erase(it);
+    (void)hipFree(p);
+}
+
 // Upload a host double vector as the context's element type.
 void* upload_vec(swm_ctx* c, const std::vector<double>& h) {
     void* p = dalloc(c, h.size() * c->esz);
@@ -138,7 +141,7 @@ void* upload_vec(swm_ctx* c, const std::vector<double>& h) {
     return p;
 }
 
-// Upload a host (Nx,Ny) double array into a pitched device array of the context's type.
+// Upload a host (Nx,Ny) double array into a pitched device plane of the context's type.
 int upload_grid(swm_ctx* c, void* dst, const double* src) {
     const size_t n = (size_t)c->Nx * c->Ny;
     if (c->fp32) {
@@ -169,12 +172,14 @@ void* alloc_grid(swm_ctx* c) {
     return p;
 }
 
+void* plane_ptr(swm_ctx* c, int k) { return (char*)c->state + (size_t)k * c->plane * c->esz; }
+
 bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
 
 // Upload a static field.  NULL -> the constant default on every point.  If each
 // row has one value on the mask's ocean points and one on its land points the
 // field is stored as two values per row, else as a full pitched array.
-int upload_field(swm_ctx* c, int id, const double* h, double dflt, const int8_t* mask) {
+int upload_field(swm_ctx* c, FieldDev* out, const double* h, double dflt, const int8_t* mask) {
     const int Nx = c->Nx, Ny = c->Ny;
     std::vector<double> rows(2 * (size_t)Ny, dflt);
     bool compress = true;
@@ -190,23 +195,23 @@ int upload_field(swm_ctx* c, int id, const double* h, double dflt, const int8_t*
         }
     }
     if (compress) {
-        c->fields[id][1] = upload_vec(c, rows);
-        if (!c->fields[id][1]) return fail(c, "field upload failed", 3);
-        c->fields[id][0] = nullptr;
+        out->rows = upload_vec(c, rows);
+        out->full = nullptr;
+        if (!out->rows) return fail(c, "field upload failed", 3);
     } else {
         void* p = alloc_grid(c);
         if (!p) return fail(c, "out of device memory", 3);
         int rc = upload_grid(c, p, h);
         if (rc) return rc;
-        c->fields[id][0] = p;
-        c->fields[id][1] = nullptr;
+        out->full = p;
+        out->rows = nullptr;
     }
     return 0;
 }
 
 template <typename T>
-Fld<T> F(const swm_ctx* c, int id) {
-    return Fld<T>{(const T*)c->fields[id][0], (const T*)c->fields[id][1]};
+Fld<T> F(const FieldDev& f) {
+    return Fld<T>{(const T*)f.full, (const T*)f.rows};
 }
 
 // Per-row coefficient tables, each built with the reference's expression.
@@ -236,7 +241,7 @@ int build_tables(swm_ctx* c) {
         t[T_EDU][j] = A * cu[j] * dL;
         t[T_ACE][j] = A * ce[j];
         for (int ot = 1; ot <= 2; ++ot) {
-            // SWM_LateralMixing_init (src/swm_lateralmixing_module.f90:88-94), bc = 1
+            // SWM_LateralMixing_init (src/swm_lateralmixing_module.f90:88-94), bc = 1 on ocean points
             const double bc = 1.0;
             double* lu = &t[T_LMU][(j * 2 + ot - 1) * 3];
             lu[0] = -(bc / A / cu[j] / dL);
@@ -249,7 +254,7 @@ int build_tables(swm_ctx* c) {
         }
         for (int a1 = 0; a1 < 2; ++a1)
             for (int a0 = 0; a0 < 2; ++a0) {
-                // _init_p_coefficients (src/swm_lateralmixing_module.f90:135-142), bc = 1
+                // _init_p_coefficients (src/swm_lateralmixing_module.f90:135-142), bc = 1 on ocean points
                 const double bc = 1.0;
                 int ot = std::max(1, a0 + a1);
                 double* pl = &t[T_PLLC][(j * 4 + a1 * 2 + a0) * 3];   // a1 = ov(j+1), a0 = ov(j)
@@ -307,35 +312,33 @@ int build_tables(swm_ctx* c) {
     return 0;
 }
 
+// (Re)write the device Statics<T> from the context.
 template <typename T>
-StepArgs<T> make_args(swm_ctx* c) {
+int write_statics_t(swm_ctx* c) {
     const swm_desc& d = c->d;
-    StepArgs<T> a{};
+    Statics<T> a{};
     a.Nx = c->Nx; a.Ny = c->Ny; a.P = c->P;
-    const TileShape ts = kTiles[c->tile];
-    a.ntx = (c->Nx + ts.tx - 1) / ts.tx;
-    a.nty = (c->Ny + ts.ty - 1) / ts.ty;
-    a.nblk = a.ntx * a.nty;
     a.mix = (d.flags & SWM_LATERAL_MIXING) ? 1 : 0;
     a.quad = (d.flags & SWM_QUADRATIC_BOTTOM_FRICTION) ? 1 : 0;
     a.barotropic = (d.flags & SWM_BAROTROPIC) ? 1 : 0;
     a.has_tau_scale = (d.flags & SWM_TAU_SCALE) ? 1 : 0;
-    a.has_taux = d.tau_x != nullptr;
-    a.has_tauy = d.tau_y != nullptr;
-    a.has_ox = d.osc_re[0] != nullptr;
-    a.has_oy = d.osc_re[1] != nullptr;
-    a.has_oe = d.osc_re[2] != nullptr;
+    a.has_taux = c->has_taux;
+    a.has_tauy = c->has_tauy;
+    a.has_oy = d.osc_re[1] ? 1 : 0;
+    a.has_oe = d.osc_re[2] ? 1 : 0;
+    a.has_ox = d.osc_re[0] ? 1 : 0;
     a.G = (T)d.G; a.A = (T)d.A; a.dL = (T)d.dLambda; a.dT = (T)d.dTheta; a.RHO0 = (T)d.RHO0;
     a.minD = (T)d.minD; a.lbc = (T)d.lbc; a.tau_scale = (T)d.tau_scale; a.AdT = (T)(d.A * d.dTheta);
     a.eps = std::is_same<T, double>::value ? (T)2.220446049250313e-16 : (T)1.1920929e-07f;
     a.mask = c->mask;
-    a.He = F<T>(c, F_HE); a.Hu = F<T>(c, F_HU); a.Hv = F<T>(c, F_HV);
-    a.imu = F<T>(c, F_IMU); a.imv = F<T>(c, F_IMV); a.ime = F<T>(c, F_IME);
-    a.gsu = F<T>(c, F_GSU); a.gsv = F<T>(c, F_GSV);
-    a.Fxc = F<T>(c, F_FXC); a.Fyc = F<T>(c, F_FYC); a.Fec = F<T>(c, F_FEC);
-    a.taux = F<T>(c, F_TAUX); a.tauy = F<T>(c, F_TAUY);
-    a.oRx = F<T>(c, F_ORX); a.oIx = F<T>(c, F_OIX); a.oRy = F<T>(c, F_ORY); a.oIy = F<T>(c, F_OIY);
-    a.oRe = F<T>(c, F_ORE); a.oIe = F<T>(c, F_OIE);
+    a.He = F<T>(c->fields[F_HE]); a.Hu = F<T>(c->fields[F_HU]); a.Hv = F<T>(c->fields[F_HV]);
+    a.imu = F<T>(c->fields[F_IMU]); a.imv = F<T>(c->fields[F_IMV]); a.ime = F<T>(c->fields[F_IME]);
+    a.gsu = F<T>(c->fields[F_GSU]); a.gsv = F<T>(c->fields[F_GSV]);
+    a.Fxc = F<T>(c->fields[F_FXC]); a.Fyc = F<T>(c->fields[F_FYC]); a.Fec = F<T>(c->fields[F_FEC]);
+    a.taux = F<T>(c->fields[F_TAUX]); a.tauy = F<T>(c->fields[F_TAUY]);
+    a.oRx = F<T>(c->fields[F_ORX]); a.oIx = F<T>(c->fields[F_OIX]);
+    a.oRy = F<T>(c->fields[F_ORY]); a.oIy = F<T>(c->fields[F_OIY]);
+    a.oRe = F<T>(c->fields[F_ORE]); a.oIe = F<T>(c->fields[F_OIE]);
     a.cu = (const T*)c->tab[T_CU]; a.cv = (const T*)c->tab[T_CV]; a.ce = (const T*)c->tab[T_CE];
     a.cH = (const T*)c->tab[T_CH]; a.fH = (const T*)c->tab[T_FH]; a.za = (const T*)c->tab[T_ZA];
     a.edu = (const T*)c->tab[T_EDU]; a.ace = (const T*)c->tab[T_ACE];
@@ -343,42 +346,47 @@ StepArgs<T> make_args(swm_ctx* c) {
     a.pllc = (const T*)c->tab[T_PLLC]; a.pltc = (const T*)c->tab[T_PLTC];
     a.tA = (const T*)c->tab[T_TA]; a.tB = (const T*)c->tab[T_TB]; a.tC = (const T*)c->tab[T_TC];
     a.t9 = (const T*)c->tab[T_T9]; a.t11 = (const T*)c->tab[T_T11]; a.t12 = (const T*)c->tab[T_T12];
-    // state at N0 and the AB roles
-    const int n0 = c->cur, n1 = 1 - c->cur;
-    const int r0 = c->ring, r1 = (c->ring + 1) % 3, r2 = (c->ring + 2) % 3;
-    a.u = (const T*)c->u[n0]; a.v = (const T*)c->v[n0]; a.eta = (const T*)c->eta[n0];
-    a.u_n = (T*)c->u[n1]; a.v_n = (T*)c->v[n1]; a.eta_n = (T*)c->eta[n1];
-    a.Gu1 = (const T*)c->Gu[r0]; a.Gu2 = (const T*)c->Gu[r1]; a.Gu3 = (T*)c->Gu[r2];
-    a.Gv1 = (const T*)c->Gv[r0]; a.Gv2 = (const T*)c->Gv[r1]; a.Gv3 = (T*)c->Gv[r2];
-    a.Ge1 = (const T*)c->Ge[r0]; a.Ge2 = (const T*)c->Ge[r1]; a.Ge3 = (T*)c->Ge[r2];
-    for (size_t n = 0; n < c->trc.size() && n < 2; ++n) {
-        Tracer& tr = c->trc[n];
-        swm::TrcArgs<T>& ta = a.trc[n];
-        ta.kappa = (T)tr.kappa;
-        ta.CH = (const T*)tr.CH[n0]; ta.CH_new = (T*)tr.CH[n1];
-        ta.Cbuf = tr.cbuf_valid ? (const T*)tr.Cbuf : nullptr;
-        ta.G1 = (const T*)tr.G[r0]; ta.G2 = (const T*)tr.G[r1]; ta.G3 = (T*)tr.G[r2];
-        if (std::is_same<T, double>::value) {
-            ta.C0 = Fld<T>{(const T*)tr.C0d.full, (const T*)tr.C0d.rows};
-            ta.gC = Fld<T>{(const T*)tr.gCd.full, (const T*)tr.gCd.rows};
-            ta.impl = Fld<T>{(const T*)tr.impld.full, (const T*)tr.impld.rows};
-        } else {
-            ta.C0 = Fld<T>{(const T*)tr.C0f.full, (const T*)tr.C0f.rows};
-            ta.gC = Fld<T>{(const T*)tr.gCf.full, (const T*)tr.gCf.rows};
-            ta.impl = Fld<T>{(const T*)tr.implf.full, (const T*)tr.implf.rows};
-        }
-        ta.dC = (T*)c->tdiag[n][0]; ta.duhc = (T*)c->tdiag[n][1]; ta.dvhc = (T*)c->tdiag[n][2];
+    for (size_t n = 0; n < c->trc.size(); ++n) {
+        a.kappa[n] = (T)c->trc[n].kappa;
+        a.C0[n] = F<T>(c->trc[n].C0);
+        a.gC[n] = F<T>(c->trc[n].gC);
+        a.timpl[n] = F<T>(c->trc[n].impl);
+        for (int k = 0; k < 3; ++k) a.tdiag[n][k] = (T*)c->tdiag[n][k];
     }
-    return a;
+    for (int k = 0; k < 14; ++k) a.diag[k] = (T*)c->diag[k];
+    if (!c->statics && !(c->statics = dalloc(c, sizeof(Statics<T>)))) return fail(c, "out of device memory", 3);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->statics, &a, sizeof(a), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int write_statics(swm_ctx* c) { return c->fp32 ? write_statics_t<float>(c) : write_statics_t<double>(c); }
+
+template <typename T>
+Dyn<T> make_dyn(swm_ctx* c, int tile) {
+    Dyn<T> d{};
+    d.s = (const Statics<T>*)c->statics;
+    d.state = (T*)c->state;
+    d.plane = c->plane;
+    const TileShape ts = kTiles[tile];
+    d.ntx = (c->Nx + ts.tx - 1) / ts.tx;
+    d.nblk = d.ntx * ((c->Ny + ts.ty - 1) / ts.ty);
+    d.cur = c->cur;
+    d.ring = c->ring;
+    d.tstep = 1;
+    d.cbuf = 0;
+    for (size_t n = 0; n < c->trc.size(); ++n)
+        if (c->trc[n].cbuf_valid) d.cbuf |= 1 << n;
+    return d;
 }
 
 template <typename T, int TX, int TY, bool DIAG>
-void launch_tile(swm_ctx* c, const StepArgs<T>& a) {
+void launch_tile(swm_ctx* c, const Dyn<T>& d) {
     const int var = c->d.variant;
     const int nt = (int)c->trc.size();
-    dim3 grid(a.nblk), block(256);
+    dim3 grid(d.nblk), block(256);
 #define SWM_LAUNCH(V, N) \
-    hipLaunchKernelGGL((swm::step_kernel<T, V, N, TX, TY, DIAG>), grid, block, 0, c->stream, a)
+    hipLaunchKernelGGL((swm::step_kernel<T, V, N, TX, TY, DIAG>), grid, block, 0, c->stream, d)
     if (var == SWM_FULLY_NONLINEAR) {
         if (nt == 0) SWM_LAUNCH(swm::VAR_FN, 0);
         else if (nt == 1) SWM_LAUNCH(swm::VAR_FN, 1);
@@ -392,16 +400,16 @@ void launch_tile(swm_ctx* c, const StepArgs<T>& a) {
 }
 
 template <typename T, bool DIAG>
-int launch_step_t(swm_ctx* c, const StepArgs<T>& a) {
+int launch_step_t(swm_ctx* c, const Dyn<T>& d, int tile) {
     if (DIAG) {
-        launch_tile<T, 64, 8, true>(c, a);
+        launch_tile<T, 64, 8, true>(c, d);
     } else {
-        switch (c->tile) {
-        case 1: launch_tile<T, 32, 8, false>(c, a); break;
-        case 2: launch_tile<T, 64, 4, false>(c, a); break;
-        case 3: launch_tile<T, 128, 4, false>(c, a); break;
-        case 4: launch_tile<T, 32, 16, false>(c, a); break;
-        default: launch_tile<T, 64, 8, false>(c, a); break;
+        switch (tile) {
+        case 1: launch_tile<T, 32, 8, false>(c, d); break;
+        case 2: launch_tile<T, 64, 4, false>(c, d); break;
+        case 3: launch_tile<T, 128, 4, false>(c, d); break;
+        case 4: launch_tile<T, 32, 16, false>(c, d); break;
+        default: launch_tile<T, 64, 8, false>(c, d); break;
         }
     }
     HIPCHK(c, hipGetLastError());
@@ -409,25 +417,27 @@ int launch_step_t(swm_ctx* c, const StepArgs<T>& a) {
 }
 
 template <typename T>
-int step_one(swm_ctx* c) {
-    c->itt += 1;
-    StepArgs<T> a = make_args<T>(c);
-    const int tstep = (int)std::min<int64_t>(3, c->itt);
-    a.tstep = tstep;
-    a.dtc1 = (T)(c->d.dt * c->ab[tstep - 1][0]);
-    a.dtc2 = (T)(c->d.dt * c->ab[tstep - 1][1]);
-    a.dtc3 = (T)(c->d.dt * c->ab[tstep - 1][2]);
-    // the forcing used by step itt was set by SWM_forcing_update at itt-1
-    const double tf = (double)(c->itt - 1);
+void set_osc(swm_ctx* c, Dyn<T>& d, double tf) {
     for (int comp = 0; comp < 3; ++comp) {
         if (!c->d.osc_re[comp]) continue;
         const double arg = c->d.osc_omega[comp] * tf * c->d.dt;
-        const double cs = std::cos(arg), sn = std::sin(arg);
-        if (comp == 0) { a.ocx = (T)cs; a.osx = (T)sn; }
-        if (comp == 1) { a.ocy = (T)cs; a.osy = (T)sn; }
-        if (comp == 2) { a.oce = (T)cs; a.ose = (T)sn; }
+        d.oc[comp] = (T)std::cos(arg);
+        d.os[comp] = (T)std::sin(arg);
     }
-    int rc = launch_step_t<T, false>(c, a);
+}
+
+template <typename T>
+int step_one(swm_ctx* c) {
+    c->itt += 1;
+    Dyn<T> d = make_dyn<T>(c, c->tile);
+    const int tstep = (int)std::min<int64_t>(3, c->itt);
+    d.tstep = tstep;
+    d.dtc1 = (T)(c->d.dt * c->ab[tstep - 1][0]);
+    d.dtc2 = (T)(c->d.dt * c->ab[tstep - 1][1]);
+    d.dtc3 = (T)(c->d.dt * c->ab[tstep - 1][2]);
+    // the forcing used by step itt was set by SWM_forcing_update at itt-1
+    set_osc(c, d, (double)(c->itt - 1));
+    int rc = launch_step_t<T, false>(c, d, c->tile);
     if (rc) return rc;
     // model_advance: N0 <- N0p1, G(1) <- G(2), G(2) <- G(3)
     c->cur = 1 - c->cur;
@@ -446,12 +456,11 @@ void drop_graphs(swm_ctx* c) {
 // so six steps are captured once per phase and replayed as one graph.
 template <typename T>
 int step_many(swm_ctx* c, int64_t nsteps) {
-    const bool has_osc = c->d.osc_re[0] || c->d.osc_re[1] || c->d.osc_re[2];
     int64_t done = 0;
     while (done < nsteps) {
         bool cbuf = false;
         for (auto& tr : c->trc) cbuf = cbuf || tr.cbuf_valid;
-        const bool steady = c->use_graph && !has_osc && !cbuf && c->itt >= 2;
+        const bool steady = c->use_graph && !c->has_osc && !cbuf && c->itt >= 2;
         if (steady && nsteps - done >= 6) {
             const int key = c->cur * 3 + c->ring;
             if (!c->graphs[key]) {
@@ -481,29 +490,40 @@ int step_many(swm_ctx* c, int64_t nsteps) {
 
 template <typename T>
 int materialise(swm_ctx* c) {
+    bool fresh = false;
     for (int k = 0; k < 14; ++k)
-        if (!c->diag[k] && !(c->diag[k] = alloc_grid(c))) return fail(c, "out of device memory", 3);
+        if (!c->diag[k]) {
+            if (!(c->diag[k] = alloc_grid(c))) return fail(c, "out of device memory", 3);
+            fresh = true;
+        }
     for (size_t n = 0; n < c->trc.size(); ++n)
         for (int k = 0; k < 3; ++k)
-            if (!c->tdiag[n][k] && !(c->tdiag[n][k] = alloc_grid(c))) return fail(c, "out of device memory", 3);
-    const int saved_tile = c->tile;
-    c->tile = 0;   // the materialisation kernel uses the 64 x 8 tile
-    StepArgs<T> a = make_args<T>(c);
-    c->tile = saved_tile;
-    T** outs[14] = {&a.dD, &a.dDh, &a.dDu, &a.dDv, &a.dzeta, &a.dPot, &a.dEd, &a.dMU, &a.dMV,
-                    &a.dFx, &a.dFy, &a.dFe, &a.dLu, &a.dLv};
-    for (int k = 0; k < 14; ++k) *outs[k] = (T*)c->diag[k];
-    a.tstep = 1;
-    const double tf = (double)c->itt;   // forcing as left by the last SWM_forcing_update
-    for (int comp = 0; comp < 3; ++comp) {
-        if (!c->d.osc_re[comp]) continue;
-        const double arg = c->d.osc_omega[comp] * tf * c->d.dt;
-        const double cs = std::cos(arg), sn = std::sin(arg);
-        if (comp == 0) { a.ocx = (T)cs; a.osx = (T)sn; }
-        if (comp == 1) { a.ocy = (T)cs; a.osy = (T)sn; }
-        if (comp == 2) { a.oce = (T)cs; a.ose = (T)sn; }
+            if (!c->tdiag[n][k]) {
+                if (!(c->tdiag[n][k] = alloc_grid(c))) return fail(c, "out of device memory", 3);
+                fresh = true;
+            }
+    if (fresh) {
+        int rc = write_statics(c);
+        if (rc) return rc;
     }
-    return launch_step_t<T, true>(c, a);
+    Dyn<T> d = make_dyn<T>(c, 0);   // the materialisation kernel uses the 64 x 8 tile
+    // forcing as left by the last SWM_forcing_update
+    set_osc(c, d, (double)c->itt);
+    return launch_step_t<T, true>(c, d, 0);
+}
+
+int alloc_state(swm_ctx* c, int ntrc) {
+    const int planes = swm::PL_TRC + swm::PL_PER_TRC * ntrc;
+    void* p = dalloc(c, (size_t)planes * c->plane * c->esz);
+    if (!p) return fail(c, "out of device memory", 3);
+    HIPCHK(c, hipMemset(p, 0, (size_t)planes * c->plane * c->esz));
+    if (c->state) {
+        const int old = swm::PL_TRC + swm::PL_PER_TRC * (ntrc - 1);
+        HIPCHK(c, hipMemcpy(p, c->state, (size_t)old * c->plane * c->esz, hipMemcpyDeviceToDevice));
+        dfree(c, c->state);
+    }
+    c->state = p;
+    return 0;
 }
 
 }  // namespace
@@ -548,11 +568,21 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
     if (const char* e = getenv("SWM_GPU_GRAPH")) c->use_graph = atoi(e) != 0;
     const int align = c->fp32 ? 32 : 16;   // 128-byte aligned rows
     c->P = (c->Nx + align - 1) / align * align;
+    c->plane = (long long)c->P * c->Ny;
+    c->has_osc = d->osc_re[0] || d->osc_re[1] || d->osc_re[2];
+    c->has_taux = d->tau_x != nullptr;
+    c->has_tauy = d->tau_y != nullptr;
     for (int g = 0; g < 4; ++g) {
         if (!d->grid[g].ocean || !d->grid[g].cos_lat || !d->grid[g].tan_lat || !d->grid[g].f)
             return fail(c, "every grid needs ocean, cos_lat, tan_lat and f", 1);
         c->hmask[g].assign(d->grid[g].ocean, d->grid[g].ocean + (size_t)c->Nx * c->Ny);
     }
+    if (!d->grid[SWM_GRID_ETA].H) return fail(c, "grid[ETA].H is required", 1);
+    if ((d->osc_re[0] && !d->osc_im[0]) || (d->osc_re[1] && !d->osc_im[1]) || (d->osc_re[2] && !d->osc_im[2]))
+        return fail(c, "oscillating forcing needs both real and imaginary parts", 1);
+    if ((d->tau_x || d->tau_y) && d->variant != SWM_FULLY_NONLINEAR)
+        return fail(c, "per-step windstress streams are a FULLY_NONLINEAR feature; fold constant "
+                       "windstress into F_*_const for linear variants", 1);
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(c, hipEventCreate(&c->ev0));
     HIPCHK(c, hipEventCreate(&c->ev1));
@@ -563,7 +593,6 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
     c->ab[2][0] = 5.0 / 12.0; c->ab[2][1] = -16.0 / 12.0; c->ab[2][2] = 23.0 / 12.0;
     // packed masks: bit0 H, bit1 u, bit2 v, bit3 eta
     {
-        const size_t n = (size_t)c->Nx * c->Ny;
         std::vector<uint8_t> m((size_t)c->Ny * c->P, 0);
         for (int j = 0; j < c->Ny; ++j)
             for (int i = 0; i < c->Nx; ++i) {
@@ -571,51 +600,37 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
                 m[(size_t)j * c->P + i] = (uint8_t)((c->hmask[0][k] ? 1 : 0) | (c->hmask[1][k] ? 2 : 0) |
                                                     (c->hmask[2][k] ? 4 : 0) | (c->hmask[3][k] ? 8 : 0));
             }
-        (void)n;
         c->mask = (uint8_t*)dalloc(c, m.size());
         if (!c->mask) return fail(c, "out of device memory", 3);
         HIPCHK(c, hipMemcpy(c->mask, m.data(), m.size(), hipMemcpyHostToDevice));
     }
-    const int8_t *mH = c->hmask[0].data(), *mu = c->hmask[1].data(), *mv = c->hmask[2].data(),
-                 *me = c->hmask[3].data();
-    (void)mH;
+    const int8_t *mu = c->hmask[1].data(), *mv = c->hmask[2].data(), *me = c->hmask[3].data();
     int rc = 0;
-    if (!d->grid[SWM_GRID_ETA].H) return fail(c, "grid[ETA].H is required", 1);
-    if ((rc = upload_field(c, F_HE, d->grid[SWM_GRID_ETA].H, 0.0, me))) return rc;
-    if ((rc = upload_field(c, F_HU, d->grid[SWM_GRID_U].H, 0.0, mu))) return rc;
-    if ((rc = upload_field(c, F_HV, d->grid[SWM_GRID_V].H, 0.0, mv))) return rc;
-    if ((rc = upload_field(c, F_IMU, d->impl_u, 1.0, mu))) return rc;
-    if ((rc = upload_field(c, F_IMV, d->impl_v, 1.0, mv))) return rc;
-    if ((rc = upload_field(c, F_IME, d->impl_eta, 1.0, me))) return rc;
-    if ((rc = upload_field(c, F_GSU, d->gamma_sq_u, 0.0, mu))) return rc;
-    if ((rc = upload_field(c, F_GSV, d->gamma_sq_v, 0.0, mv))) return rc;
-    if ((rc = upload_field(c, F_FXC, d->F_x_const, 0.0, mu))) return rc;
-    if ((rc = upload_field(c, F_FYC, d->F_y_const, 0.0, mv))) return rc;
-    if ((rc = upload_field(c, F_FEC, d->F_eta_const, 0.0, me))) return rc;
-    if ((rc = upload_field(c, F_TAUX, d->tau_x, 0.0, mu))) return rc;
-    if ((rc = upload_field(c, F_TAUY, d->tau_y, 0.0, mv))) return rc;
-    if ((rc = upload_field(c, F_ORX, d->osc_re[0], 0.0, mu))) return rc;
-    if ((rc = upload_field(c, F_OIX, d->osc_im[0], 0.0, mu))) return rc;
-    if ((rc = upload_field(c, F_ORY, d->osc_re[1], 0.0, mv))) return rc;
-    if ((rc = upload_field(c, F_OIY, d->osc_im[1], 0.0, mv))) return rc;
-    if ((rc = upload_field(c, F_ORE, d->osc_re[2], 0.0, me))) return rc;
-    if ((rc = upload_field(c, F_OIE, d->osc_im[2], 0.0, me))) return rc;
-    if ((d->osc_re[0] && !d->osc_im[0]) || (d->osc_re[1] && !d->osc_im[1]) || (d->osc_re[2] && !d->osc_im[2]))
-        return fail(c, "oscillating forcing needs both real and imaginary parts", 1);
-    if ((d->tau_x || d->tau_y) && d->variant != SWM_FULLY_NONLINEAR)
-        return fail(c, "per-step windstress streams are a FULLY_NONLINEAR feature; fold constant "
-                       "windstress into F_*_const for linear variants", 1);
+    struct { int id; const double* h; double dflt; const int8_t* m; } ups[] = {
+        {F_HE, d->grid[SWM_GRID_ETA].H, 0.0, me}, {F_HU, d->grid[SWM_GRID_U].H, 0.0, mu},
+        {F_HV, d->grid[SWM_GRID_V].H, 0.0, mv},   {F_IMU, d->impl_u, 1.0, mu},
+        {F_IMV, d->impl_v, 1.0, mv},              {F_IME, d->impl_eta, 1.0, me},
+        {F_GSU, d->gamma_sq_u, 0.0, mu},          {F_GSV, d->gamma_sq_v, 0.0, mv},
+        {F_FXC, d->F_x_const, 0.0, mu},           {F_FYC, d->F_y_const, 0.0, mv},
+        {F_FEC, d->F_eta_const, 0.0, me},         {F_TAUX, d->tau_x, 0.0, mu},
+        {F_TAUY, d->tau_y, 0.0, mv},              {F_ORX, d->osc_re[0], 0.0, mu},
+        {F_OIX, d->osc_im[0], 0.0, mu},           {F_ORY, d->osc_re[1], 0.0, mv},
+        {F_OIY, d->osc_im[1], 0.0, mv},           {F_ORE, d->osc_re[2], 0.0, me},
+        {F_OIE, d->osc_im[2], 0.0, me}};
+    for (auto& u : ups)
+        if ((rc = upload_field(c, &c->fields[u.id], u.h, u.dflt, u.m))) return rc;
     if ((rc = build_tables(c))) return rc;
-    for (int l = 0; l < 2; ++l) {
-        if (!(c->u[l] = alloc_grid(c)) || !(c->v[l] = alloc_grid(c)) || !(c->eta[l] = alloc_grid(c)))
-            return fail(c, "out of device memory", 3);
-    }
-    for (int l = 0; l < 3; ++l) {
-        if (!(c->Gu[l] = alloc_grid(c)) || !(c->Gv[l] = alloc_grid(c)) || !(c->Ge[l] = alloc_grid(c)))
-            return fail(c, "out of device memory", 3);
-    }
+    if ((rc = alloc_state(c, 0))) return rc;
     if (!(c->scratch = alloc_grid(c))) return fail(c, "out of device memory", 3);
-    return 0;
+    // host pointers in the descriptor are only valid during this call
+    for (int g = 0; g < 4; ++g) c->d.grid[g] = swm_grid_desc{};
+    c->d.impl_u = c->d.impl_v = c->d.impl_eta = c->d.gamma_sq_u = c->d.gamma_sq_v = nullptr;
+    c->d.F_x_const = c->d.F_y_const = c->d.F_eta_const = c->d.tau_x = c->d.tau_y = nullptr;
+    for (int k = 0; k < 3; ++k) {
+        c->d.osc_re[k] = d->osc_re[k] ? (const double*)1 : nullptr;   // presence flag only
+        c->d.osc_im[k] = nullptr;
+    }
+    return write_statics(c);
 }
 
 int swm_gpu_upload_state(swm_ctx* c, const double* u, const double* v, const double* eta, const double* G_u,
@@ -623,18 +638,20 @@ int swm_gpu_upload_state(swm_ctx* c, const double* u, const double* v, const dou
     if (!c) return 1;
     if (!u || !v || !eta) return fail(c, "u, v and eta are required", 1);
     if (itt < 0) return fail(c, "itt must be >= 0", 1);
-    const size_t n = (size_t)c->Nx * c->Ny;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     drop_graphs(c);
+    const size_t n = (size_t)c->Nx * c->Ny;
     c->cur = 0;
     c->ring = 0;
     int rc;
-    if ((rc = upload_grid(c, c->u[0], u)) || (rc = upload_grid(c, c->v[0], v)) || (rc = upload_grid(c, c->eta[0], eta)))
+    if ((rc = upload_grid(c, plane_ptr(c, swm::PL_U), u)) || (rc = upload_grid(c, plane_ptr(c, swm::PL_V), v)) ||
+        (rc = upload_grid(c, plane_ptr(c, swm::PL_E), eta)))
         return rc;
     std::vector<double> zero(n, 0.0);
     for (int l = 0; l < 3; ++l) {
-        if ((rc = upload_grid(c, c->Gu[l], G_u ? G_u + l * n : zero.data()))) return rc;
-        if ((rc = upload_grid(c, c->Gv[l], G_v ? G_v + l * n : zero.data()))) return rc;
-        if ((rc = upload_grid(c, c->Ge[l], G_eta ? G_eta + l * n : zero.data()))) return rc;
+        if ((rc = upload_grid(c, plane_ptr(c, swm::PL_GU + l), G_u ? G_u + l * n : zero.data()))) return rc;
+        if ((rc = upload_grid(c, plane_ptr(c, swm::PL_GV + l), G_v ? G_v + l * n : zero.data()))) return rc;
+        if ((rc = upload_grid(c, plane_ptr(c, swm::PL_GE + l), G_eta ? G_eta + l * n : zero.data()))) return rc;
     }
     c->itt = itt;
     return 0;
@@ -642,49 +659,34 @@ int swm_gpu_upload_state(swm_ctx* c, const double* u, const double* v, const dou
 
 int swm_gpu_add_tracer(swm_ctx* c, const swm_tracer_desc* t, int32_t* tracer_id) {
     if (!c || !t) return 1;
-    if (c->trc.size() >= 2) return fail(c, "at most 2 tracers are fused into the step kernel", 4);
+    if ((int)c->trc.size() >= kMaxTracers) return fail(c, "at most 2 tracers are fused into the step kernel", 4);
     if (!t->CH || !t->C) return fail(c, "tracer CH and C are required", 1);
     if (c->d.variant != SWM_FULLY_NONLINEAR)
         return fail(c, "tracers need FULLY_NONLINEAR (h = D); the linear D alias makes C = CH/0 on land", 4);
-    const size_t n = (size_t)c->Nx * c->Ny;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     drop_graphs(c);
-    Tracer tr;
-    tr.kappa = t->kappa_h;
+    const size_t n = (size_t)c->Nx * c->Ny;
+    const int nt = (int)c->trc.size();
     int rc;
-    for (int l = 0; l < 2; ++l)
-        if (!(tr.CH[l] = alloc_grid(c))) return fail(c, "out of device memory", 3);
-    for (int l = 0; l < 3; ++l)
-        if (!(tr.G[l] = alloc_grid(c))) return fail(c, "out of device memory", 3);
-    if (!(tr.Cbuf = alloc_grid(c))) return fail(c, "out of device memory", 3);
-    // the tracer's buffers follow the context's current ping-pong / ring roles
-    if ((rc = upload_grid(c, tr.CH[c->cur], t->CH))) return rc;
-    if ((rc = upload_grid(c, tr.Cbuf, t->C))) return rc;
+    if ((rc = alloc_state(c, nt + 1))) return rc;
+    const int base = swm::PL_TRC + swm::PL_PER_TRC * nt;
+    // the tracer's planes follow the context's current ping-pong / ring roles
+    if ((rc = upload_grid(c, plane_ptr(c, base + c->cur), t->CH))) return rc;
+    if ((rc = upload_grid(c, plane_ptr(c, base + 5), t->C))) return rc;
     std::vector<double> zero(n, 0.0);
     for (int l = 0; l < 3; ++l)
-        if ((rc = upload_grid(c, tr.G[(c->ring + l) % 3], t->G_CH ? t->G_CH + l * n : zero.data()))) return rc;
+        if ((rc = upload_grid(c, plane_ptr(c, base + 2 + (c->ring + l) % 3), t->G_CH ? t->G_CH + l * n : zero.data())))
+            return rc;
+    Tracer tr;
+    tr.kappa = t->kappa_h;
     tr.cbuf_valid = true;
     const int8_t* me = c->hmask[3].data();
-    const int F0 = F_COUNT;   // temporary slot in fields[] is not used: upload directly
-    (void)F0;
-    auto up = [&](const double* h, double dflt, void** full, void** rows) -> int {
-        void* save[2] = {c->fields[0][0], c->fields[0][1]};
-        int r = upload_field(c, 0, h, dflt, me);
-        *full = c->fields[0][0];
-        *rows = c->fields[0][1];
-        c->fields[0][0] = save[0];
-        c->fields[0][1] = save[1];
-        return r;
-    };
-    void *f0, *r0, *f1, *r1, *f2, *r2;
-    if ((rc = up(t->C0, 0.0, &f0, &r0)) || (rc = up(t->gamma_C, 0.0, &f1, &r1)) || (rc = up(t->impl, 1.0, &f2, &r2)))
+    if ((rc = upload_field(c, &tr.C0, t->C0, 0.0, me)) || (rc = upload_field(c, &tr.gC, t->gamma_C, 0.0, me)) ||
+        (rc = upload_field(c, &tr.impl, t->impl, 1.0, me)))
         return rc;
-    if (c->fp32) {
-        tr.C0f = {(float*)f0, (float*)r0}; tr.gCf = {(float*)f1, (float*)r1}; tr.implf = {(float*)f2, (float*)r2};
-    } else {
-        tr.C0d = {(double*)f0, (double*)r0}; tr.gCd = {(double*)f1, (double*)r1}; tr.impld = {(double*)f2, (double*)r2};
-    }
     c->trc.push_back(tr);
-    if (tracer_id) *tracer_id = (int32_t)c->trc.size() - 1;
+    if ((rc = write_statics(c))) return rc;
+    if (tracer_id) *tracer_id = nt;
     return 0;
 }
 
@@ -721,10 +723,10 @@ int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_o
     const void* src = nullptr;
     if (tracer_id >= 0) {
         if (tracer_id >= (int)c->trc.size()) return fail(c, "no such tracer", 1);
-        Tracer& tr = c->trc[tracer_id];
-        if (s == "CH") src = tr.CH[n0];
-        else if (s == "G_CH") src = tr.G[r1];      // newest tendency (G(3) == G(2) after the shift)
-        else if (s == "G_CH_M1") src = tr.G[r0];
+        const int base = swm::PL_TRC + swm::PL_PER_TRC * tracer_id;
+        if (s == "CH") src = plane_ptr(c, base + n0);
+        else if (s == "G_CH") src = plane_ptr(c, base + 2 + r1);   // newest tendency (G(3) == G(2) after the shift)
+        else if (s == "G_CH_M1") src = plane_ptr(c, base + 2 + r0);
         else if (s == "C" || s == "UHC" || s == "VHC") {
             int rc = c->fp32 ? materialise<float>(c) : materialise<double>(c);
             if (rc) return rc;
@@ -732,15 +734,15 @@ int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_o
         } else {
             return fail(c, "unknown tracer field " + s, 1);
         }
-    } else if (s == "SWM_U" || s == "U") src = c->u[n0];
-    else if (s == "SWM_V" || s == "V") src = c->v[n0];
-    else if (s == "SWM_ETA" || s == "ETA") src = c->eta[n0];
-    else if (s == "G_U") src = c->Gu[r1];
-    else if (s == "G_V") src = c->Gv[r1];
-    else if (s == "G_ETA") src = c->Ge[r1];
-    else if (s == "G_U_M1") src = c->Gu[r0];
-    else if (s == "G_V_M1") src = c->Gv[r0];
-    else if (s == "G_ETA_M1") src = c->Ge[r0];
+    } else if (s == "SWM_U" || s == "U") src = plane_ptr(c, swm::PL_U + n0);
+    else if (s == "SWM_V" || s == "V") src = plane_ptr(c, swm::PL_V + n0);
+    else if (s == "SWM_ETA" || s == "ETA") src = plane_ptr(c, swm::PL_E + n0);
+    else if (s == "G_U") src = plane_ptr(c, swm::PL_GU + r1);
+    else if (s == "G_V") src = plane_ptr(c, swm::PL_GV + r1);
+    else if (s == "G_ETA") src = plane_ptr(c, swm::PL_GE + r1);
+    else if (s == "G_U_M1") src = plane_ptr(c, swm::PL_GU + r0);
+    else if (s == "G_V_M1") src = plane_ptr(c, swm::PL_GV + r0);
+    else if (s == "G_ETA_M1") src = plane_ptr(c, swm::PL_GE + r0);
     else {
         static const char* names[14] = {"SWM_D", "SWM_DH", "SWM_DU", "SWM_DV", "SWM_RELVORT", "SWM_POT", "SWM_EDENS",
                                         "SWM_MU", "SWM_MV", "F_X", "F_Y", "F_ETA", "SWM_LATMIX_U", "SWM_LATMIX_V"};
@@ -920,12 +922,14 @@ int swm_gpu_divergence(swm_ctx* c, double* host_div) {
     const int nb = (n + 255) / 256;
     if (c->fp32)
         hipLaunchKernelGGL(swm::divergence_kernel<float>, dim3(nb), dim3(256), 0, c->stream, c->Nx, c->Ny, c->P,
-                           (const float*)c->u[c->cur], (const float*)c->v[c->cur], c->mask, (const float*)c->dza,
+                           (const float*)plane_ptr(c, swm::PL_U + c->cur), (const float*)plane_ptr(c, swm::PL_V + c->cur),
+                           c->mask, (const float*)c->dza,
                            (const float*)c->tab[T_CV], (const float*)c->tab[T_CE], (float)(c->d.A * c->d.dTheta),
                            (float)c->d.lbc, (float*)c->scratch);
     else
         hipLaunchKernelGGL(swm::divergence_kernel<double>, dim3(nb), dim3(256), 0, c->stream, c->Nx, c->Ny, c->P,
-                           (const double*)c->u[c->cur], (const double*)c->v[c->cur], c->mask, (const double*)c->dza,
+                           (const double*)plane_ptr(c, swm::PL_U + c->cur),
+                           (const double*)plane_ptr(c, swm::PL_V + c->cur), c->mask, (const double*)c->dza,
                            (const double*)c->tab[T_CV], (const double*)c->tab[T_CE], c->d.A * c->d.dTheta, c->d.lbc,
                            (double*)c->scratch);
     HIPCHK(c, hipGetLastError());

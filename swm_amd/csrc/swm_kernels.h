@@ -37,32 +37,15 @@ __device__ __forceinline__ T fld(const Fld<T>& f, size_t off, int j, bool ocean)
     return f.full ? f.full[off] : f.rows[2 * j + (ocean ? 0 : 1)];
 }
 
+// Static inputs of the step kernel, resident in device memory and read through
+// one pointer (scalar loads on use), so the per-launch argument block stays
+// small and the kernel does not spill uniform values out of SGPRs.
 template <typename T>
-struct TrcArgs {
-    T kappa;
-    const T* CH;      // level N0
-    T* CH_new;        // level N0p1
-    const T* Cbuf;    // concentration held by the host (first step after hand-over), else null
-    const T* G1;      // AB history, oldest
-    const T* G2;      // middle
-    T* G3;            // newest (written)
-    Fld<T> C0, gC, impl;
-    T *dC, *duhc, *dvhc;   // diagnostic outputs
-};
-
-template <typename T>
-struct StepArgs {
-    int Nx, Ny, P, ntx, nty, nblk;
-    int tstep;              // min(3, itt)
+struct Statics {
+    int Nx, Ny, P;
     int mix, quad, barotropic, has_tau_scale;
     int has_taux, has_tauy, has_ox, has_oy, has_oe;
     T G, A, dL, dT, RHO0, minD, lbc, tau_scale, AdT, eps;
-    T dtc1, dtc2, dtc3;     // dt*ab_coeff(ti, tstep)
-    T ocx, osx, ocy, osy, oce, ose;   // cos/sin(omega*itt*dt) of the oscillating streams
-    const T *u, *v, *eta;
-    T *u_n, *v_n, *eta_n;
-    const T *Gu1, *Gu2, *Gv1, *Gv2, *Ge1, *Ge2;
-    T *Gu3, *Gv3, *Ge3;
     const uint8_t* mask;
     Fld<T> He, Hu, Hv, imu, imv, ime, gsu, gsv, Fxc, Fyc, Fec, taux, tauy, oRx, oIx, oRy, oIy, oRe, oIe;
     // per-row tables
@@ -76,9 +59,30 @@ struct StepArgs {
     const T* pltc;   // [Ny][ou(j)][ou(j-1)][3]
     const T *tA, *tB, *tC;   // [Ny][2] tracer advection magnitudes, denominator 1 / 2
     const T *t9, *t11, *t12; // [Ny] tracer diffusion magnitudes
-    TrcArgs<T> trc[2];
+    // tracers
+    T kappa[2];
+    Fld<T> C0[2], gC[2], timpl[2];
     // diagnostic outputs (materialisation kernel only)
-    T *dD, *dDh, *dDu, *dDv, *dzeta, *dPot, *dEd, *dMU, *dMV, *dFx, *dFy, *dFe, *dLu, *dLv;
+    T* diag[14];      // D Dh Du Dv zeta Pot EDens MU MV F_x F_y F_eta latmix_u latmix_v
+    T* tdiag[2][3];   // C uhc vhc per tracer
+};
+
+// State planes in one allocation: u, v, eta at two levels, three AB slots per
+// field, then per tracer CH (two levels), G_CH (three slots) and the C buffer.
+enum : int { PL_U = 0, PL_V = 2, PL_E = 4, PL_GU = 6, PL_GV = 9, PL_GE = 12, PL_TRC = 15, PL_PER_TRC = 6 };
+
+// Per-launch arguments (by value).
+template <typename T>
+struct Dyn {
+    const Statics<T>* s;
+    T* state;
+    long long plane;     // elements per plane (Ny * P)
+    int ntx, nblk;
+    int cur, ring;       // ping-pong level of N0; ring slot of G(1)
+    int tstep;           // min(3, itt)
+    int cbuf;            // bit n: tracer n reads its C buffer (first step after hand-over)
+    T dtc1, dtc2, dtc3;  // dt*ab_coeff(ti, tstep)
+    T oc[3], os[3];      // cos/sin(omega*itt*dt) of the oscillating streams (x, y, eta)
 };
 
 // weight = 1 / max(sum of masks, 1) of the *_noland interpolators
@@ -128,7 +132,7 @@ __device__ __forceinline__ T trc_adv(bool neg, int nummask, int den, const T* ta
 
 template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG>
 __global__ void __launch_bounds__(256)
-step_kernel(const StepArgs<T> a) {
+step_kernel(const Dyn<T> d) {
     constexpr int NT = 256;
     constexpr int EW = TX + 2;
     constexpr int EH = TY + 2;
@@ -139,12 +143,26 @@ step_kernel(const StepArgs<T> a) {
     __shared__ T s_CH[NTR][E], s_C[NTR][E];
     __shared__ uint8_t s_m[E];
 
+    const Statics<T>& a = *d.s;
     const int tid = threadIdx.x;
-    const int t = xcd_tile(blockIdx.x, a.nblk);
-    const int i0 = (t % a.ntx) * TX;
-    const int j0 = (t / a.ntx) * TY;
+    const int t = xcd_tile(blockIdx.x, d.nblk);
+    const int i0 = (t % d.ntx) * TX;
+    const int j0 = (t / d.ntx) * TY;
     const int Nx = a.Nx, Ny = a.Ny, P = a.P;
     const bool FN = (VAR == VAR_FN);
+    // planes of this step's roles
+    T* const st = d.state;
+    const long long pl = d.plane;
+    const int r0 = d.ring, r1 = d.ring == 2 ? 0 : d.ring + 1, r2 = d.ring == 0 ? 2 : d.ring - 1;
+    const T* const su = st + (PL_U + d.cur) * pl;
+    const T* const sv = st + (PL_V + d.cur) * pl;
+    const T* const se = st + (PL_E + d.cur) * pl;
+    const T* const Gu1 = st + (PL_GU + r0) * pl;
+    const T* const Gu2 = st + (PL_GU + r1) * pl;
+    const T* const Gv1 = st + (PL_GV + r0) * pl;
+    const T* const Gv2 = st + (PL_GV + r1) * pl;
+    const T* const Ge1 = st + (PL_GE + r0) * pl;
+    const T* const Ge2 = st + (PL_GE + r1) * pl;
 
     // ---- prefetch this thread's AB history (independent of the LDS phases)
     T pg[NO][6];
@@ -156,11 +174,15 @@ step_kernel(const StepArgs<T> a) {
             const int i = i0 + o % TX, j = j0 + o / TX;
             if (o < TX * TY && i < Nx && j < Ny) {
                 const size_t off = (size_t)j * P + i;
-                pg[q][0] = a.Gu1[off]; pg[q][1] = a.Gu2[off];
-                pg[q][2] = a.Gv1[off]; pg[q][3] = a.Gv2[off];
-                pg[q][4] = a.Ge1[off]; pg[q][5] = a.Ge2[off];
+                pg[q][0] = Gu1[off]; pg[q][1] = Gu2[off];
+                pg[q][2] = Gv1[off]; pg[q][3] = Gv2[off];
+                pg[q][4] = Ge1[off]; pg[q][5] = Ge2[off];
 #pragma unroll
-                for (int n = 0; n < NTRC; ++n) { pt[q][n][0] = a.trc[n].G1[off]; pt[q][n][1] = a.trc[n].G2[off]; }
+                for (int n = 0; n < NTRC; ++n) {
+                    const T* tb = st + (PL_TRC + PL_PER_TRC * n + 2) * pl;
+                    pt[q][n][0] = tb[r0 * pl + off];
+                    pt[q][n][1] = tb[r1 * pl + off];
+                }
             }
         }
     }
@@ -171,10 +193,10 @@ step_kernel(const StepArgs<T> a) {
         const int gi = wrap(i0 + lx - 1, Nx), gj = wrap(j0 + ly - 1, Ny);
         const size_t off = (size_t)gj * P + gi;
         const uint8_t m = a.mask[off];
-        const T eta = a.eta[off];
+        const T eta = se[off];
         s_m[e] = m;
-        s_u[e] = a.u[off];
-        s_v[e] = a.v[off];
+        s_u[e] = su[off];
+        s_v[e] = sv[off];
         s_eta[e] = eta;
         const T He = fld(a.He, off, gj, OE(m) != 0);
         T D;
@@ -183,9 +205,11 @@ step_kernel(const StepArgs<T> a) {
         s_D[e] = D;
 #pragma unroll
         for (int n = 0; n < NTRC; ++n) {
-            const T ch = a.trc[n].CH[off];
+            const T* tb = st + (PL_TRC + PL_PER_TRC * n) * pl;
+            const T ch = tb[d.cur * pl + off];
             s_CH[n][e] = ch;
-            s_C[n][e] = a.trc[n].Cbuf ? a.trc[n].Cbuf[off] : ch / D;   // C = CH / h (TRC_tracer_advance)
+            // C = CH / h (TRC_tracer_advance), except the host-held C right after a hand-over
+            s_C[n][e] = ((d.cbuf >> n) & 1) ? tb[5 * pl + off] : ch / D;
         }
     }
     __syncthreads();
@@ -230,17 +254,17 @@ step_kernel(const StepArgs<T> a) {
                 plt = Dh * ((c[0] * (s_v[e] - s_v[kw]) + c[1] * s_u[e]) + c[2] * s_u[ks]);
             }
             s_plt[e] = plt;
-            if (interior) { a.dDh[doff] = Dh; a.dzeta[doff] = zeta; a.dPot[doff] = Pot; }
+            if (interior) { a.diag[1][doff] = Dh; a.diag[4][doff] = zeta; a.diag[5][doff] = Pot; }
         }
         if (lx >= 1) {   // computeMassFluxes: MU = Du * u on ocean u
             const T Du = interp_noland2(s_D[e], s_D[e - 1], OE(m), OE(s_m[e - 1]));
             s_MU[e] = OU(m) ? Du * s_u[e] : T(0);
-            if (interior) { a.dDu[doff] = Du; a.dMU[doff] = s_MU[e]; }
+            if (interior) { a.diag[2][doff] = Du; a.diag[7][doff] = s_MU[e]; }
         }
         if (ly >= 1) {   // MV = Dv * v on ocean v
             const T Dv = interp_noland2(s_D[e], s_D[e - EW], OE(m), OE(s_m[e - EW]));
             s_MV[e] = OV(m) ? Dv * s_v[e] : T(0);
-            if (interior) { a.dDv[doff] = Dv; a.dMV[doff] = s_MV[e]; }
+            if (interior) { a.diag[3][doff] = Dv; a.diag[8][doff] = s_MV[e]; }
         }
         if (lx <= TX && ly <= TY) {
             const int ke = e + 1, kn = e + EW;
@@ -263,7 +287,7 @@ step_kernel(const StepArgs<T> a) {
                 pll = s_D[e] * ((c[0] * (s_u[ke] - s_u[e]) + c[1] * s_v[kn]) + c[2] * s_v[e]);
             }
             s_pll[e] = pll;
-            if (interior) { a.dEd[doff] = Ed; a.dD[doff] = s_D[e]; }
+            if (interior) { a.diag[6][doff] = Ed; a.diag[0][doff] = s_D[e]; }
         }
     }
     __syncthreads();
@@ -286,19 +310,19 @@ step_kernel(const StepArgs<T> a) {
 
         // SWM_forcing_update (src/swm_forcing_module.f90:135-170)
         T Fe = fld(a.Fec, off, j, oe);
-        if (a.has_oe) Fe = Fe + (T(2) * (fld(a.oRe, off, j, oe) * a.oce - fld(a.oIe, off, j, oe) * a.ose)) * T(oe);
+        if (a.has_oe) Fe = Fe + (T(2) * (fld(a.oRe, off, j, oe) * d.oc[2] - fld(a.oIe, off, j, oe) * d.os[2])) * T(oe);
         T Fx = fld(a.Fxc, off, j, ou);
         if (a.has_taux && ou) {
             const T tx = fld(a.taux, off, j, true);
             Fx = Fx + (a.has_tau_scale ? a.tau_scale * tx : tx) / (a.RHO0 * Du);
         }
-        if (a.has_ox) Fx = Fx + (T(2) * (fld(a.oRx, off, j, ou) * a.ocx - fld(a.oIx, off, j, ou) * a.osx)) * T(ou);
+        if (a.has_ox) Fx = Fx + (T(2) * (fld(a.oRx, off, j, ou) * d.oc[0] - fld(a.oIx, off, j, ou) * d.os[0])) * T(ou);
         T Fy = fld(a.Fyc, off, j, ov);
         if (a.has_tauy && ov) {
             const T ty = fld(a.tauy, off, j, true);
             Fy = Fy + (a.has_tau_scale ? a.tau_scale * ty : ty) / (a.RHO0 * Dv);
         }
-        if (a.has_oy) Fy = Fy + (T(2) * (fld(a.oRy, off, j, ov) * a.ocy - fld(a.oIy, off, j, ov) * a.osy)) * T(ov);
+        if (a.has_oy) Fy = Fy + (T(2) * (fld(a.oRy, off, j, ov) * d.oc[1] - fld(a.oIy, off, j, ov) * d.os[1])) * T(ov);
 
         // SWM_timestep_nonlinear (src/swm_timestep_module.f90:349-443)
         T Ge = T(0), Gu = T(0), Gv = T(0), Lu = T(0), Lv = T(0);
@@ -349,14 +373,14 @@ step_kernel(const StepArgs<T> a) {
         }
 
         if (DIAG) {
-            a.dFx[off] = Fx; a.dFy[off] = Fy; a.dFe[off] = Fe;
-            a.dLu[off] = Lu; a.dLv[off] = Lv;
+            a.diag[9][off] = Fx; a.diag[10][off] = Fy; a.diag[11][off] = Fe;
+            a.diag[12][off] = Lu; a.diag[13][off] = Lv;
 #pragma unroll
             for (int n = 0; n < NTRC; ++n) {
                 const T C = s_C[n][k];
-                a.trc[n].dC[off] = C;
-                a.trc[n].duhc[off] = s_MU[k] * T(0.5) * (C + s_C[n][kw]);
-                a.trc[n].dvhc[off] = s_MV[k] * T(0.5) * (C + s_C[n][ks]);
+                a.tdiag[n][0][off] = C;
+                a.tdiag[n][1][off] = s_MU[k] * T(0.5) * (C + s_C[n][kw]);
+                a.tdiag[n][2][off] = s_MV[k] * T(0.5) * (C + s_C[n][ks]);
             }
             continue;
         }
@@ -367,18 +391,18 @@ step_kernel(const StepArgs<T> a) {
         const T Gv3 = ov ? Gv : pg[q][3];
         const T Ge3 = oe ? Ge : pg[q][5];
         T un = s_u[k], vn = s_v[k], en = s_eta[k];
-        un = un + a.dtc1 * pg[q][0];
-        vn = vn + a.dtc1 * pg[q][2];
-        en = en + a.dtc1 * pg[q][4];
-        if (a.tstep >= 2) {
-            un = un + a.dtc2 * pg[q][1];
-            vn = vn + a.dtc2 * pg[q][3];
-            en = en + a.dtc2 * pg[q][5];
+        un = un + d.dtc1 * pg[q][0];
+        vn = vn + d.dtc1 * pg[q][2];
+        en = en + d.dtc1 * pg[q][4];
+        if (d.tstep >= 2) {
+            un = un + d.dtc2 * pg[q][1];
+            vn = vn + d.dtc2 * pg[q][3];
+            en = en + d.dtc2 * pg[q][5];
         }
-        if (a.tstep >= 3) {
-            un = un + a.dtc3 * Gu3;
-            vn = vn + a.dtc3 * Gv3;
-            en = en + a.dtc3 * Ge3;
+        if (d.tstep >= 3) {
+            un = un + d.dtc3 * Gu3;
+            vn = vn + d.dtc3 * Gv3;
+            en = en + d.dtc3 * Ge3;
         }
         un = un / fld(a.imu, off, j, ou);
         vn = vn / fld(a.imv, off, j, ov);
@@ -387,13 +411,17 @@ step_kernel(const StepArgs<T> a) {
             const T lim = a.minD - fld(a.He, off, j, oe);
             if (en < lim) en = lim;
         }
-        a.u_n[off] = un; a.v_n[off] = vn; a.eta_n[off] = en;
-        a.Gu3[off] = Gu3; a.Gv3[off] = Gv3; a.Ge3[off] = Ge3;
+        st[(PL_U + 1 - d.cur) * pl + off] = un;
+        st[(PL_V + 1 - d.cur) * pl + off] = vn;
+        st[(PL_E + 1 - d.cur) * pl + off] = en;
+        st[(PL_GU + r2) * pl + off] = Gu3;
+        st[(PL_GV + r2) * pl + off] = Gv3;
+        st[(PL_GE + r2) * pl + off] = Ge3;
 
         // TRC_tracer_incr + TRC_tracer_integrate (src/tracer_module.f90:154-232)
 #pragma unroll
         for (int n = 0; n < NTRC; ++n) {
-            const TrcArgs<T>& tr = a.trc[n];
+            T* const tb = st + (PL_TRC + PL_PER_TRC * n) * pl;
             const T CH = s_CH[n][k], CHe = s_CH[n][ke], CHw = s_CH[n][kw], CHn = s_CH[n][kn], CHs = s_CH[n][ks];
             const T C = s_C[n][k], Ce = s_C[n][ke], Cw = s_C[n][kw], Cn = s_C[n][kn], Cs = s_C[n][ks];
             // tracer velocities are the master fields u = 0 + SWM_u (model_advance)
@@ -403,7 +431,7 @@ step_kernel(const StepArgs<T> a) {
             const int oue = OU(me), ovn = OV(mn);
             T diff = T(0);
             if (oe) {
-                const T kap = tr.kappa;
+                const T kap = a.kappa[n];
                 const T hue = interp_noland2(s_D[ke], s_D[k], oee, oe);
                 const T hvn = interp_noland2(s_D[kn], s_D[k], oen, oe);
                 const T c9 = oue ? a.t9[j] : T(0.0);
@@ -430,14 +458,14 @@ step_kernel(const StepArgs<T> a) {
             g = g + (CH * vmn) * c7;
             g = g + (CH * vm) * c8;
             g = g + diff;
-            g = g - (fld(tr.gC, off, j, oe) * s_D[k]) * (C - fld(tr.C0, off, j, oe));
+            g = g - (fld(a.gC[n], off, j, oe) * s_D[k]) * (C - fld(a.C0[n], off, j, oe));
             g = g + forcing;
-            T chn = CH + a.dtc1 * pt[q][n][0];
-            if (a.tstep >= 2) chn = chn + a.dtc2 * pt[q][n][1];
-            if (a.tstep >= 3) chn = chn + a.dtc3 * g;
-            chn = chn / fld(tr.impl, off, j, oe);
-            tr.CH_new[off] = chn;
-            tr.G3[off] = g;
+            T chn = CH + d.dtc1 * pt[q][n][0];
+            if (d.tstep >= 2) chn = chn + d.dtc2 * pt[q][n][1];
+            if (d.tstep >= 3) chn = chn + d.dtc3 * g;
+            chn = chn / fld(a.timpl[n], off, j, oe);
+            tb[(1 - d.cur) * pl + off] = chn;
+            tb[(2 + r2) * pl + off] = g;
         }
     }
 }
