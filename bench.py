@@ -16,6 +16,10 @@ where the algorithmic bytes per cell-step are
 = 161 B (H, impl, windstress are row-uniform in this workload and travel as
 per-row tables).
 
+roofline.measured_copy_GBs is a plain 1 GiB device-to-device copy timed on
+the same device (read + write bytes), the practical HBM ceiling beside the
+8 TB/s peak.
+
 cpu_baseline: the reference-order C restatement (oracle/) on the host cores,
 timed on the same workload for a bounded number of steps.
 """
@@ -51,6 +55,26 @@ def read_traffic(precision: str):
     return None
 
 
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 4):
+    """Device-to-device copy of nbytes (read nbytes + write nbytes), timed with
+    events; returns (GB/s counting both directions, seconds per copy)."""
+    import torch
+    n = nbytes // 8
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / reps
+    del x, y
+    return 2 * n * 8 / sec / 1e9, sec
+
+
 def cpu_baseline(s, budget_s: float):
     import oracle
     o = oracle.Oracle(s)
@@ -79,6 +103,8 @@ def main(argv=None):
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy four times (counter calibration) and exit")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: every run here uses exactly one MI355X; --gpus must be 1",
@@ -90,6 +116,10 @@ def main(argv=None):
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        gbs, sec = copy_bandwidth(256 << 20)
+        print(json.dumps({"copy_bytes_each_way": 256 << 20, "copies": 5, "GB/s": gbs, "s_per_copy": sec}))
+        return 0
     from swm_amd import configs, gpu
 
     s = configs.c2_nonlinear(seed=0, N=args.n, dt=args.dt, nsteps=args.warmup + args.steps)
@@ -107,6 +137,7 @@ def main(argv=None):
     finite = bool(np.isfinite(eta).all())
     g.close()
 
+    copy_gbs, _ = copy_bandwidth()
     cells = s.Nx * s.Ny
     value = cells * args.steps / (total_ms * 1e-3)
     bpc = BYTES_PER_CELL[args.precision]
@@ -132,7 +163,8 @@ def main(argv=None):
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(args.precision),
-            "bytes_per_cell_step": bpc,
+            "bytes_per_cell_step": bpc, "measured_copy_GBs": copy_gbs,
+            "frac_of_measured_copy": achieved / copy_gbs,
         },
     }
     if not args.no_cpu:

@@ -151,6 +151,18 @@ int swm_gpu_sor_fixed(swm_ctx *c, const double *B, double *chi, double omega, do
  * hands to the SOR (src/calc_lib.f90:500-525).  Copies div to the host. */
 int swm_gpu_divergence(swm_ctx *c, double *host_div);
 
+/* Conservation sums of the current device state in FP64 with compensated
+ * (Neumaier) accumulation, also for FP32 storage.  No counterpart in the
+ * reference, which has no conservation diagnostics; it serves callers that
+ * check an FP32 run's mass and energy drift against an FP64 one.  Over ocean
+ * eta points, weighted by cos(lat_eta):
+ *   out[0] = sum eta
+ *   out[1] = sum 0.5*D*(0.5*(u(i)^2+u(i+1)^2) + 0.5*(v(j)^2+v(j+1)^2)) + 0.5*G*eta^2
+ *   out[2] = sum CH of tracer tracer_id (0 when tracer_id < 0)
+ * D = eta + H_eta for FULLY_NONLINEAR, H_eta otherwise.  Multiply by
+ * A^2*dLambda*dTheta for volumes.  Blocks until the result is on the host. */
+int swm_gpu_sums(swm_ctx *c, int32_t tracer_id, double *out);
+
 /* Time spent in the last swm_gpu_step call's kernels (ms, device clock) and
  * the average duration of the step kernel (ms), for roofline accounting. */
 int swm_gpu_last_timing(const swm_ctx *c, double *total_ms, double *step_kernel_ms);

@@ -27,10 +27,14 @@ using swm::Statics;
 
 namespace {
 
-// Step-kernel tile shapes (outputs per 256-thread block).  Index 0 is the
-// default; the others stay selectable (SWM_GPU_TILE) for tuning.
-struct TileShape { int tx, ty; };
-constexpr TileShape kTiles[] = {{64, 8}, {32, 8}, {64, 4}, {128, 4}, {32, 16}};
+// Step-kernel shapes, selectable with SWM_GPU_TILE for tuning.  march = 0:
+// the LDS tile kernel with tx x ty outputs per 256-thread block.  march = 1:
+// the marching-wavefront kernel, four wavefronts per block, each owning 62
+// columns and ty rows; ty = 0 picks the smallest ty for which every
+// wavefront of the grid is resident at once (one round, no tail).
+struct TileShape { int march, tx, ty; };
+constexpr TileShape kTiles[] = {{0, 64, 4}, {0, 32, 8}, {0, 64, 8}, {0, 128, 4}, {0, 32, 16},
+                                {1, 62, 0}, {1, 62, 8}, {1, 62, 16}, {1, 62, 4}};
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 constexpr int kMaxTracers = 2;
 
@@ -82,6 +86,11 @@ struct swm_ctx {
     std::vector<Tracer> trc;
     int tile = 0;
     bool use_graph = true;
+    int num_cu = 0;              // compute units of the device
+    double* sums_buf = nullptr;  // conservation-sum partials and result
+    double* ce64 = nullptr;      // cos(lat_eta) rows in FP64
+    std::vector<double> ce_host;
+    long march_slots = 0;        // resident wavefronts of the marching kernel (0: not queried)
     hipGraphExec_t graphs[6] = {};   // steady-state 6-step cycles, keyed by (cur, ring)
     // diagnostics scratch
     void* diag[14] = {};
@@ -363,6 +372,35 @@ int write_statics_t(swm_ctx* c) {
 int write_statics(swm_ctx* c) { return c->fp32 ? write_statics_t<float>(c) : write_statics_t<double>(c); }
 
 template <typename T>
+int march_blocks_per_cu(swm_ctx* c) {
+    const int var = c->d.variant;
+    const int nt = (int)c->trc.size();
+    int nb = 0;
+    hipError_t e = hipSuccess;
+#define SWM_OCC(V, N) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, swm::march_kernel<T, V, N>, 256, 0)
+    if (var == SWM_FULLY_NONLINEAR) {
+        if (nt == 0) SWM_OCC(swm::VAR_FN, 0);
+        else if (nt == 1) SWM_OCC(swm::VAR_FN, 1);
+        else SWM_OCC(swm::VAR_FN, 2);
+    } else {
+        if (nt == 0) SWM_OCC(swm::VAR_SOR, 0);
+        else if (nt == 1) SWM_OCC(swm::VAR_SOR, 1);
+        else SWM_OCC(swm::VAR_SOR, 2);
+    }
+#undef SWM_OCC
+    return e == hipSuccess ? nb : 0;
+}
+
+// Rows per wavefront of the marching kernel: the smallest count (at least 4)
+// whose grid fits the device's resident wavefront slots in one round.
+int march_rows(swm_ctx* c, int ntx) {
+    if (c->march_slots <= 0) return 8;
+    const long segs = std::max<long>(1, c->march_slots / ntx);
+    const int ty = (int)((c->Ny + segs - 1) / segs);
+    return std::max(4, ty);
+}
+
+template <typename T>
 Dyn<T> make_dyn(swm_ctx* c, int tile) {
     Dyn<T> d{};
     d.s = (const Statics<T>*)c->statics;
@@ -370,7 +408,11 @@ Dyn<T> make_dyn(swm_ctx* c, int tile) {
     d.plane = c->plane;
     const TileShape ts = kTiles[tile];
     d.ntx = (c->Nx + ts.tx - 1) / ts.tx;
-    d.nblk = d.ntx * ((c->Ny + ts.ty - 1) / ts.ty);
+    if (ts.march && !ts.ty && c->march_slots <= 0)
+        c->march_slots = (long)march_blocks_per_cu<T>(c) * 4 * c->num_cu;
+    d.ty = ts.ty ? ts.ty : march_rows(c, d.ntx);
+    const int nty = (c->Ny + d.ty - 1) / d.ty;
+    d.nblk = d.ntx * (ts.march ? (nty + 3) / 4 : nty);
     d.cur = c->cur;
     d.ring = c->ring;
     d.tstep = 1;
@@ -399,6 +441,24 @@ void launch_tile(swm_ctx* c, const Dyn<T>& d) {
 #undef SWM_LAUNCH
 }
 
+template <typename T>
+void launch_march(swm_ctx* c, const Dyn<T>& d) {
+    const int var = c->d.variant;
+    const int nt = (int)c->trc.size();
+    dim3 grid(d.nblk), block(256);
+#define SWM_LAUNCH(V, N) hipLaunchKernelGGL((swm::march_kernel<T, V, N>), grid, block, 0, c->stream, d)
+    if (var == SWM_FULLY_NONLINEAR) {
+        if (nt == 0) SWM_LAUNCH(swm::VAR_FN, 0);
+        else if (nt == 1) SWM_LAUNCH(swm::VAR_FN, 1);
+        else SWM_LAUNCH(swm::VAR_FN, 2);
+    } else {
+        if (nt == 0) SWM_LAUNCH(swm::VAR_SOR, 0);
+        else if (nt == 1) SWM_LAUNCH(swm::VAR_SOR, 1);
+        else SWM_LAUNCH(swm::VAR_SOR, 2);
+    }
+#undef SWM_LAUNCH
+}
+
 template <typename T, bool DIAG>
 int launch_step_t(swm_ctx* c, const Dyn<T>& d, int tile) {
     if (DIAG) {
@@ -406,10 +466,11 @@ int launch_step_t(swm_ctx* c, const Dyn<T>& d, int tile) {
     } else {
         switch (tile) {
         case 1: launch_tile<T, 32, 8, false>(c, d); break;
-        case 2: launch_tile<T, 64, 4, false>(c, d); break;
+        case 2: launch_tile<T, 64, 8, false>(c, d); break;
         case 3: launch_tile<T, 128, 4, false>(c, d); break;
         case 4: launch_tile<T, 32, 16, false>(c, d); break;
-        default: launch_tile<T, 64, 8, false>(c, d); break;
+        case 5: case 6: case 7: case 8: launch_march<T>(c, d); break;
+        default: launch_tile<T, 64, 4, false>(c, d); break;
         }
     }
     HIPCHK(c, hipGetLastError());
@@ -506,7 +567,7 @@ int materialise(swm_ctx* c) {
         int rc = write_statics(c);
         if (rc) return rc;
     }
-    Dyn<T> d = make_dyn<T>(c, 0);   // the materialisation kernel uses the 64 x 8 tile
+    Dyn<T> d = make_dyn<T>(c, 2);   // the materialisation kernel uses the 64 x 8 tile (kTiles[2])
     // forcing as left by the last SWM_forcing_update
     set_osc(c, d, (double)c->itt);
     return launch_step_t<T, true>(c, d, 0);
@@ -584,6 +645,11 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
         return fail(c, "per-step windstress streams are a FULLY_NONLINEAR feature; fold constant "
                        "windstress into F_*_const for linear variants", 1);
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    {
+        int dev = 0;
+        HIPCHK(c, hipGetDevice(&dev));
+        HIPCHK(c, hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
     HIPCHK(c, hipEventCreate(&c->ev0));
     HIPCHK(c, hipEventCreate(&c->ev1));
     // time_integration_init (src/time_integration_module.f90:44-51)
@@ -622,6 +688,7 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
     if ((rc = build_tables(c))) return rc;
     if ((rc = alloc_state(c, 0))) return rc;
     if (!(c->scratch = alloc_grid(c))) return fail(c, "out of device memory", 3);
+    c->ce_host.assign(d->grid[SWM_GRID_ETA].cos_lat, d->grid[SWM_GRID_ETA].cos_lat + c->Ny);
     // host pointers in the descriptor are only valid during this call
     for (int g = 0; g < 4; ++g) c->d.grid[g] = swm_grid_desc{};
     c->d.impl_u = c->d.impl_v = c->d.impl_eta = c->d.gamma_sq_u = c->d.gamma_sq_v = nullptr;
@@ -685,6 +752,7 @@ int swm_gpu_add_tracer(swm_ctx* c, const swm_tracer_desc* t, int32_t* tracer_id)
         (rc = upload_field(c, &tr.impl, t->impl, 1.0, me)))
         return rc;
     c->trc.push_back(tr);
+    c->march_slots = 0;   // the kernel instance (and its occupancy) depends on the tracer count
     if ((rc = write_statics(c))) return rc;
     if (tracer_id) *tracer_id = nt;
     return 0;
@@ -935,6 +1003,43 @@ int swm_gpu_divergence(swm_ctx* c, double* host_div) {
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return download_grid(c, host_div, c->scratch);
+}
+
+int swm_gpu_sums(swm_ctx* c, int32_t tracer_id, double* out) {
+    if (!c || !out) return 1;
+    if (tracer_id >= (int32_t)c->trc.size()) return fail(c, "tracer id out of range", 1);
+    const long n = (long)c->Nx * c->Ny;
+    const int nb = (int)std::min<long>(1024, (n + 255) / 256);
+    if (!c->sums_buf) {
+        c->sums_buf = (double*)dalloc(c, sizeof(double) * (6 * 1024 + 3));
+        c->ce64 = (double*)dalloc(c, sizeof(double) * c->Ny);
+        if (!c->sums_buf || !c->ce64) return fail(c, "out of device memory", 3);
+        HIPCHK(c, hipMemcpy(c->ce64, c->ce_host.data(), sizeof(double) * c->Ny, hipMemcpyHostToDevice));
+    }
+    const int fn = c->d.variant == SWM_FULLY_NONLINEAR;
+    const int ch_plane = tracer_id >= 0 ? swm::PL_TRC + swm::PL_PER_TRC * tracer_id + c->cur : -1;
+    double* partial = c->sums_buf;
+    double* dout = c->sums_buf + 6 * 1024;
+    if (c->fp32)
+        hipLaunchKernelGGL(swm::sums_kernel<float>, dim3(nb), dim3(256), 0, c->stream,
+                           (const Statics<float>*)c->statics, fn, (const float*)plane_ptr(c, swm::PL_E + c->cur),
+                           (const float*)plane_ptr(c, swm::PL_U + c->cur), (const float*)plane_ptr(c, swm::PL_V + c->cur),
+                           ch_plane >= 0 ? (const float*)plane_ptr(c, ch_plane) : nullptr, (const double*)c->ce64,
+                           partial);
+    else
+        hipLaunchKernelGGL(swm::sums_kernel<double>, dim3(nb), dim3(256), 0, c->stream,
+                           (const Statics<double>*)c->statics, fn, (const double*)plane_ptr(c, swm::PL_E + c->cur),
+                           (const double*)plane_ptr(c, swm::PL_U + c->cur),
+                           (const double*)plane_ptr(c, swm::PL_V + c->cur),
+                           ch_plane >= 0 ? (const double*)plane_ptr(c, ch_plane) : nullptr, (const double*)c->ce64,
+                           partial);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(swm::sums_final, dim3(1), dim3(256), 0, c->stream, (const double*)partial, nb, dout);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(out, dout, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (tracer_id < 0) out[2] = 0.0;
+    return 0;
 }
 
 const char* swm_gpu_last_error(const swm_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }

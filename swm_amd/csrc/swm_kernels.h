@@ -78,6 +78,7 @@ struct Dyn {
     T* state;
     long long plane;     // elements per plane (Ny * P)
     int ntx, nblk;
+    int ty;              // rows per wavefront (marching kernel)
     int cur, ring;       // ping-pong level of N0; ring slot of G(1)
     int tstep;           // min(3, itt)
     int cbuf;            // bit n: tracer n reads its C buffer (first step after hand-over)
@@ -470,6 +471,399 @@ step_kernel(const Dyn<T> d) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Marching-wavefront step kernel.
+//
+// Each 64-lane wavefront owns a strip of 64 columns (62 outputs plus one halo
+// lane on each side) and a segment of d.ty output rows.  It walks down its rows
+// one per iteration, keeps the three rows j-1, j, j+1 of the state and of the
+// derived fields in registers, takes horizontal neighbours with full-wavefront
+// DPP shifts, and loads the next row and the AB history one iteration ahead of
+// use.  No LDS and no barriers; every derived field is computed once per
+// (column, row) of the strip.  The arithmetic is the same expression for
+// expression as step_kernel, so both kernels give bit-identical results.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_left_i(int x) {    // value held by lane L-1 (wave_shr:1)
+    return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int lane_right_i(int x) {   // value held by lane L+1 (wave_shl:1)
+    return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ double lane_left(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = lane_left_i((int)(b & 0xffffffff)), hi = lane_left_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_right(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = lane_right_i((int)(b & 0xffffffff)), hi = lane_right_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float lane_left(float x) { return __int_as_float(lane_left_i(__float_as_int(x))); }
+__device__ __forceinline__ float lane_right(float x) { return __int_as_float(lane_right_i(__float_as_int(x))); }
+
+// Loads for the marching kernel: per-lane values through a global pointer with
+// a 32-bit lane index (base address stays in SGPRs), and uniform values of the
+// read-only per-row tables through the constant address space (scalar loads).
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p, unsigned i) {
+    return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void stg(T* p, unsigned i, T v) {
+    ((__attribute__((address_space(1))) T*)p)[i] = v;
+}
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p, size_t i) {
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+// fld() for a uniform row: ro = row offset, j = row index
+template <typename T>
+__device__ __forceinline__ T fld_r(const Fld<T>& f, size_t ro, unsigned gi, int j, bool ocean) {
+    if (f.full) return ldg(f.full + ro, gi);
+    const T o = ldc(f.rows, 2 * (size_t)j), l = ldc(f.rows, 2 * (size_t)j + 1);
+    return ocean ? o : l;
+}
+// trc_adv() for a uniform row
+template <typename T>
+__device__ __forceinline__ T trc_adv_r(bool neg, int nummask, int den, const T* tab, int j) {
+    const T v1 = ldc(tab, 2 * (size_t)j), v2 = ldc(tab, 2 * (size_t)j + 1);
+    if (den == 0) return __builtin_nan("");
+    if (!nummask) return neg ? T(-0.0) : T(0.0);
+    const T v = den == 1 ? v1 : v2;
+    return neg ? -v : v;
+}
+
+template <typename T, int NTR>
+struct MRaw {            // one loaded row at this lane
+    int m;
+    T u, v, eta, He;
+    T CH[NTR], Cb[NTR];
+};
+
+template <typename T, int NTR>
+struct MRow {            // state and derived fields of one row at this lane
+    int m;
+    T u, v, eta, D, Du, MU, Dv, MV, Pot, plt, Ed, pll;
+    T C[NTR], CH[NTR];
+};
+
+template <typename T, int VAR, int NTRC>
+__global__ void __launch_bounds__(256)
+march_kernel(const Dyn<T> d) {
+    constexpr int NTR = NTRC > 0 ? NTRC : 1;
+    const bool FN = (VAR == VAR_FN);
+    const Statics<T>& a = *d.s;
+    const int Nx = a.Nx, Ny = a.Ny, P = a.P;
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform per wavefront
+    const int b = xcd_tile(blockIdx.x, d.nblk);
+    const int strip = b % d.ntx, grp = b / d.ntx;
+    const int TY = d.ty;
+    const int j0 = (grp * 4 + wid) * TY;
+    if (j0 >= Ny) return;                      // uniform per wavefront
+    const int jend = j0 + TY < Ny ? j0 + TY : Ny;
+    const int i = strip * 62 + lane - 1;
+    const unsigned gi = (unsigned)wrap(i, Nx);
+    const bool out_lane = lane >= 1 && lane <= 62 && i < Nx;
+
+    T* const st = d.state;
+    const long long pl = d.plane;
+    const int r0 = d.ring, r1 = d.ring == 2 ? 0 : d.ring + 1, r2 = d.ring == 0 ? 2 : d.ring - 1;
+    const T* const su = st + (PL_U + d.cur) * pl;
+    const T* const sv = st + (PL_V + d.cur) * pl;
+    const T* const se = st + (PL_E + d.cur) * pl;
+
+    auto load_raw = [&](int k, MRaw<T, NTR>& r) {
+        const int gk = wrap(k, Ny);
+        const size_t ro = (size_t)gk * P;
+        r.m = ldg(a.mask + ro, gi);
+        r.u = ldg(su + ro, gi);
+        r.v = ldg(sv + ro, gi);
+        r.eta = ldg(se + ro, gi);
+        r.He = fld_r(a.He, ro, gi, gk, OE(r.m) != 0);
+#pragma unroll
+        for (int n = 0; n < NTRC; ++n) {
+            const T* tb = st + (PL_TRC + PL_PER_TRC * n) * pl + ro;
+            r.CH[n] = ldg(tb + d.cur * pl, gi);
+            r.Cb[n] = ((d.cbuf >> n) & 1) ? ldg(tb + 5 * pl, gi) : T(0);
+        }
+    };
+    auto load_hist = [&](int k, T* g, T (*gt)[2]) {
+        const T* const b = st + (size_t)k * P;
+        g[0] = ldg(b + (PL_GU + r0) * pl, gi); g[1] = ldg(b + (PL_GU + r1) * pl, gi);
+        g[2] = ldg(b + (PL_GV + r0) * pl, gi); g[3] = ldg(b + (PL_GV + r1) * pl, gi);
+        g[4] = ldg(b + (PL_GE + r0) * pl, gi); g[5] = ldg(b + (PL_GE + r1) * pl, gi);
+#pragma unroll
+        for (int n = 0; n < NTRC; ++n) {
+            const T* tb = b + (PL_TRC + PL_PER_TRC * n + 2) * pl;
+            gt[n][0] = ldg(tb + r0 * pl, gi);
+            gt[n][1] = ldg(tb + r1 * pl, gi);
+        }
+    };
+
+    MRaw<T, NTR> N;
+    load_raw(j0 - 1, N);
+    // One row of the march: K receives row k, P1 holds row k-1, P2 row k-2.
+    auto row = [&](const int k, MRow<T, NTR>& K, MRow<T, NTR>& P1, MRow<T, NTR>& P2) {
+        const int gk = wrap(k, Ny);
+        // ---- row k: take the prefetched values, start the loads of row k+1
+        K.m = N.m; K.u = N.u; K.v = N.v; K.eta = N.eta;
+        const T He = N.He;
+        T Cb[NTR];
+#pragma unroll
+        for (int n = 0; n < NTRC; ++n) { K.CH[n] = N.CH[n]; Cb[n] = N.Cb[n]; }
+        if (k < jend) load_raw(k + 1, N);
+        // AB history of row k-1, the row this call writes
+        T g[6], gt[NTR][2];
+        if (k > j0) load_hist(k - 1, g, gt);
+
+        // ---- row k: fields local to the row
+        K.D = FN ? (OE(K.m) ? (K.eta + He) : T(1)) : He;   // computeD; linear variants alias H_eta
+#pragma unroll
+        for (int n = 0; n < NTRC; ++n)   // C = CH / h, or the host-held C right after a hand-over
+            K.C[n] = ((d.cbuf >> n) & 1) ? Cb[n] : K.CH[n] / K.D;
+        const int KmL = lane_left_i(K.m);
+        const T KDL = lane_left(K.D);
+        K.Du = interp_noland2(K.D, KDL, OE(K.m), OE(KmL));     // eta2u_noland
+        K.MU = OU(K.m) ? K.Du * K.u : T(0);                    // computeMassFluxes
+
+        if (k >= j0) {
+            const int gkm = gk == 0 ? Ny - 1 : gk - 1;
+            // ---- row k: fields that need row k-1
+            K.Dv = interp_noland2(K.D, P1.D, OE(K.m), OE(P1.m));   // eta2v_noland
+            K.MV = OV(K.m) ? K.Dv * K.v : T(0);
+            const int P1mL = lane_left_i(P1.m);
+            const T P1DL = lane_left(P1.D);
+            // Dh = interp4(D, eta2H_noland): (i,j), (i-1,j-1), (i-1,j), (i,j-1)
+            const int m1 = OE(K.m), m2 = OE(P1mL), m3 = OE(KmL), m4 = OE(P1.m);
+            const T w = noland_w<T>(m1 + m2 + m3 + m4);
+            const T Dh = (((m1 ? w : T(0)) * K.D + (m2 ? w : T(0)) * P1DL) + (m3 ? w : T(0)) * KDL) +
+                         (m4 ? w : T(0)) * P1.D;
+            const T KvL = lane_left(K.v);
+            T zeta = T(0);
+            if (FN) {   // vorticityFromVelocities2D
+                T za_ = T(0), zb = T(0);
+                if (OV(KmL) + OV(K.m) != 0) {
+                    const T z0 = ldc(a.za, 2 * (size_t)gk), z1 = ldc(a.za, 2 * (size_t)gk + 1);
+                    za_ = (OH(K.m) ? z0 : z1) * (K.v - KvL);
+                }
+                if (OU(K.m) + OU(P1.m) != 0) {
+                    const T bcH = OH(K.m) ? T(1) : a.lbc;
+                    zb = bcH * (ldc(a.cu, gk) * K.u - ldc(a.cu, gkm) * P1.u) / a.AdT;
+                }
+                zeta = za_ - zb / ldc(a.cH, gk);
+            }
+            const T pD = Dh > a.eps ? Dh : a.eps;                      // computePotVort
+            K.Pot = FN ? (ldc(a.fH, gk) + zeta) / pD : ldc(a.fH, gk) / pD;
+            T plt = T(0);                                              // P_lambda-theta on ocean H
+            if (a.mix && OH(K.m)) {
+                const T* c = a.pltc + (size_t)gk * 12;
+                const unsigned q = (OU(K.m) * 2 + OU(P1.m)) * 3;
+                plt = Dh * ((ldg(c, q) * (K.v - KvL) + ldg(c, q + 1) * K.u) + ldg(c, q + 2) * P1.u);
+            }
+            K.plt = plt;
+            // ---- row k-1: fields that need row k
+            const T P1uR = lane_right(P1.u);
+            T Ed = T(0);                                               // computeEdens
+            if (OE(P1.m)) {
+                if (FN) {
+                    Ed = ((T(0.5) * (P1uR * P1uR) + T(0.5) * (P1.u * P1.u)) +
+                          (T(0.5) * (K.v * K.v) + T(0.5) * (P1.v * P1.v))) / T(2) + a.G * P1.eta;
+                } else {
+                    Ed = a.G * P1.eta;
+                }
+            }
+            P1.Ed = Ed;
+            T pll = T(0);                                              // P_lambda-lambda on ocean eta
+            if (a.mix && OE(P1.m)) {
+                const T* c = a.pllc + (size_t)gkm * 12;
+                const unsigned q = (OV(K.m) * 2 + OV(P1.m)) * 3;
+                pll = P1.D * ((ldg(c, q) * (P1uR - P1.u) + ldg(c, q + 1) * K.v) + ldg(c, q + 2) * P1.v);
+            }
+            P1.pll = pll;
+
+            // ---- outputs of row j = k-1 from rows j-1 (P2), j (P1), j+1 (K)
+            const int j = k - 1;
+            if (j >= j0) {
+                // neighbour values, taken with every lane active
+                const int P1mR = lane_right_i(P1.m);
+                const T P1MUR = lane_right(P1.MU), P2MUR = lane_right(P2.MU);
+                const T P1MVL = lane_left(P1.MV), KMVL = lane_left(K.MV);
+                const T P1EdL = lane_left(P1.Ed), P1pllL = lane_left(P1.pll);
+                const T P1PotR = lane_right(P1.Pot), P1pltR = lane_right(P1.plt);
+                T P1vL = T(0), P2uR = T(0);
+                if (a.quad) { P1vL = lane_left(P1.v); P2uR = lane_right(P2.u); }
+                T P1DR = T(0), CHe[NTR], CHw[NTR], Ce[NTR], Cw[NTR];
+                if (NTRC > 0) P1DR = lane_right(P1.D);
+#pragma unroll
+                for (int n = 0; n < NTRC; ++n) {
+                    CHe[n] = lane_right(P1.CH[n]); CHw[n] = lane_left(P1.CH[n]);
+                    Ce[n] = lane_right(P1.C[n]); Cw[n] = lane_left(P1.C[n]);
+                }
+                if (out_lane) {
+                    const size_t ro = (size_t)j * P;
+                    const int jp = gk;   // j+1, wrapped
+                    const int m = P1.m;
+                    const int oe = OE(m), ou = OU(m), ov = OV(m), oh = OH(m);
+                    const T Du = P1.Du, Dv = P1.Dv;
+                    // SWM_forcing_update
+                    T Fe = fld_r(a.Fec, ro, gi, j, oe);
+                    if (a.has_oe) Fe = Fe + (T(2) * (fld_r(a.oRe, ro, gi, j, oe) * d.oc[2] - fld_r(a.oIe, ro, gi, j, oe) * d.os[2])) * T(oe);
+                    T Fx = fld_r(a.Fxc, ro, gi, j, ou);
+                    if (a.has_taux && ou) {
+                        const T tx = fld_r(a.taux, ro, gi, j, true);
+                        Fx = Fx + (a.has_tau_scale ? a.tau_scale * tx : tx) / (a.RHO0 * Du);
+                    }
+                    if (a.has_ox) Fx = Fx + (T(2) * (fld_r(a.oRx, ro, gi, j, ou) * d.oc[0] - fld_r(a.oIx, ro, gi, j, ou) * d.os[0])) * T(ou);
+                    T Fy = fld_r(a.Fyc, ro, gi, j, ov);
+                    if (a.has_tauy && ov) {
+                        const T ty = fld_r(a.tauy, ro, gi, j, true);
+                        Fy = Fy + (a.has_tau_scale ? a.tau_scale * ty : ty) / (a.RHO0 * Dv);
+                    }
+                    if (a.has_oy) Fy = Fy + (T(2) * (fld_r(a.oRy, ro, gi, j, ov) * d.oc[1] - fld_r(a.oIy, ro, gi, j, ov) * d.os[1])) * T(ov);
+
+                    // SWM_timestep_nonlinear
+                    T Ge = T(0), Gu = T(0), Gv = T(0);
+                    if (oe) {
+                        const T t1 = -(P1MUR - P1.MU) / a.dL;
+                        const T t2 = (K.MV * ldc(a.cv, jp) - P1.MV * ldc(a.cv, j)) / a.dT;
+                        Ge = (t1 - t2) / ldc(a.ace, j) + Fe;
+                    }
+                    if (ou) {
+                        const T P2u = T(0.5) * K.Pot + T(0.5) * P1.Pot;
+                        const T M4 = ((T(0.25) * K.MV + T(0.25) * P1MVL) + T(0.25) * KMVL) + T(0.25) * P1.MV;
+                        T gg = P2u * M4;
+                        gg = gg - ((P1.Ed - P1EdL) / ldc(a.edu, j));
+                        if (a.quad) {
+                            const T vi = ((T(0.25) * K.v + T(0.25) * P1vL) + T(0.25) * KvL) + T(0.25) * P1.v;
+                            T qf = fld_r(a.gsu, ro, gi, j, true) * sqrt(P1.u * P1.u + (vi * vi)) * P1.u;
+                            if (a.barotropic) qf = qf / fld_r(a.Hu, ro, gi, j, true);
+                            gg = gg - qf;
+                        }
+                        if (a.mix) {
+                            const int ot = (oh + OH(K.m)) < 1 ? 1 : (oh + OH(K.m));
+                            const T* c = a.lmu + (size_t)j * 6;
+                            const T c0 = ot == 1 ? ldc(c, 0) : ldc(c, 3), c1 = ot == 1 ? ldc(c, 1) : ldc(c, 4),
+                                    c2 = ot == 1 ? ldc(c, 2) : ldc(c, 5);
+                            gg = gg + ((c0 * (P1.pll - P1pllL) + c1 * K.plt) + c2 * P1.plt) / Du;
+                        }
+                        Gu = gg + Fx;
+                    }
+                    if (ov) {
+                        const T P2v = T(0.5) * P1PotR + T(0.5) * P1.Pot;
+                        const T M4 = ((T(0.25) * P1MUR + T(0.25) * P2.MU) + T(0.25) * P1.MU) + T(0.25) * P2MUR;
+                        T gg = -(P2v * M4);
+                        gg = gg - ((P1.Ed - P2.Ed) / a.AdT);
+                        if (a.quad) {
+                            const T ui = ((T(0.25) * P1uR + T(0.25) * P2.u) + T(0.25) * P1.u) + T(0.25) * P2uR;
+                            T qf = fld_r(a.gsv, ro, gi, j, true) * sqrt(P1.v * P1.v + (ui * ui)) * P1.v;
+                            if (a.barotropic) qf = qf / fld_r(a.Hv, ro, gi, j, true);
+                            gg = gg - qf;
+                        }
+                        if (a.mix) {
+                            const int ot = (oe + OE(P2.m)) < 1 ? 1 : (oe + OE(P2.m));
+                            const T* c = a.lmv + (size_t)j * 6;
+                            const T c0 = ot == 1 ? ldc(c, 0) : ldc(c, 3), c1 = ot == 1 ? ldc(c, 1) : ldc(c, 4),
+                                    c2 = ot == 1 ? ldc(c, 2) : ldc(c, 5);
+                            gg = gg + ((c0 * (P1pltR - P1.plt) + c1 * P1.pll) + c2 * P2.pll) / Dv;
+                        }
+                        Gv = gg + Fy;
+                    }
+                    // integrate_AB; land keeps the previous newest tendency
+                    const T Gu3 = ou ? Gu : g[1];
+                    const T Gv3 = ov ? Gv : g[3];
+                    const T Ge3 = oe ? Ge : g[5];
+                    T un = P1.u, vn = P1.v, en = P1.eta;
+                    un = un + d.dtc1 * g[0];
+                    vn = vn + d.dtc1 * g[2];
+                    en = en + d.dtc1 * g[4];
+                    if (d.tstep >= 2) {
+                        un = un + d.dtc2 * g[1];
+                        vn = vn + d.dtc2 * g[3];
+                        en = en + d.dtc2 * g[5];
+                    }
+                    if (d.tstep >= 3) {
+                        un = un + d.dtc3 * Gu3;
+                        vn = vn + d.dtc3 * Gv3;
+                        en = en + d.dtc3 * Ge3;
+                    }
+                    un = un / fld_r(a.imu, ro, gi, j, ou);
+                    vn = vn / fld_r(a.imv, ro, gi, j, ov);
+                    en = en / fld_r(a.ime, ro, gi, j, oe);
+                    if (FN) {   // where (eta < minD - H) eta = minD - H
+                        const T lim = a.minD - fld_r(a.He, ro, gi, j, oe);
+                        if (en < lim) en = lim;
+                    }
+                    T* const wb = st + ro;
+                    stg(wb + (PL_U + 1 - d.cur) * pl, gi, un);
+                    stg(wb + (PL_V + 1 - d.cur) * pl, gi, vn);
+                    stg(wb + (PL_E + 1 - d.cur) * pl, gi, en);
+                    stg(wb + (PL_GU + r2) * pl, gi, Gu3);
+                    stg(wb + (PL_GV + r2) * pl, gi, Gv3);
+                    stg(wb + (PL_GE + r2) * pl, gi, Ge3);
+
+                    // TRC_tracer_incr + TRC_tracer_integrate
+#pragma unroll
+                    for (int n = 0; n < NTRC; ++n) {
+                        T* const tb = st + (PL_TRC + PL_PER_TRC * n) * pl + ro;
+                        const T CH = P1.CH[n], CHn = K.CH[n], CHs = P2.CH[n];
+                        const T C = P1.C[n], Cn = K.C[n], Cs = P2.C[n];
+                        // tracer velocities are the master fields u = 0 + SWM_u (model_advance)
+                        const T um = T(0) + P1.u, ume = T(0) + P1uR, vm = T(0) + P1.v, vmn = T(0) + K.v;
+                        const int oee = OE(P1mR), oew = OE(P1mL), oen = OE(K.m), oes = OE(P2.m);
+                        const int oue = OU(P1mR), ovn = OV(K.m);
+                        T diff = T(0);
+                        if (oe) {
+                            const T kap = a.kappa[n];
+                            const T hue = interp_noland2(P1DR, P1.D, oee, oe);
+                            const T hvn = K.Dv;   // interp of D(i,j+1), D(i,j) with the same weights
+                            const T c9 = oue ? ldc(a.t9, j) : T(0.0);
+                            const T c10 = ou ? -ldc(a.t9, j) : T(-0.0);
+                            const T c11 = ovn ? ldc(a.t11, j) : T(0.0);
+                            const T c12 = ov ? -ldc(a.t12, j) : T(-0.0);
+                            diff = ((((kap * hue) * (Ce[n] - C)) * c9 + ((kap * Du) * (C - Cw[n])) * c10) +
+                                    ((kap * hvn) * (Cn - C)) * c11) + ((kap * Dv) * (C - Cs)) * c12;
+                        }
+                        const T forcing = C * Fe;
+                        const T c1 = trc_adv_r<T>(true, oue & oee, oee + oe, a.tA, j);
+                        const T c2 = trc_adv_r<T>(false, ou & oew, oew + oe, a.tA, j);
+                        const T c3 = trc_adv_r<T>(true, ovn & oen, oen + oe, a.tB, j);
+                        const T c4 = trc_adv_r<T>(false, ov & oes, oes + oe, a.tC, j);
+                        const T c5 = trc_adv_r<T>(true, oue & oe, oee + oe, a.tA, j);
+                        const T c6 = trc_adv_r<T>(false, ou & oe, oew + oe, a.tA, j);
+                        const T c7 = trc_adv_r<T>(true, ovn & oe, oen + oe, a.tB, j);
+                        const T c8 = trc_adv_r<T>(false, ov & oe, oes + oe, a.tC, j);
+                        T gg = (CHe[n] * ume) * c1 + (CHw[n] * um) * c2;
+                        gg = gg + (CHn * vmn) * c3;
+                        gg = gg + (CHs * vm) * c4;
+                        gg = gg + (CH * ume) * c5;
+                        gg = gg + (CH * um) * c6;
+                        gg = gg + (CH * vmn) * c7;
+                        gg = gg + (CH * vm) * c8;
+                        gg = gg + diff;
+                        gg = gg - (fld_r(a.gC[n], ro, gi, j, oe) * P1.D) * (C - fld_r(a.C0[n], ro, gi, j, oe));
+                        gg = gg + forcing;
+                        T chn = CH + d.dtc1 * gt[n][0];
+                        if (d.tstep >= 2) chn = chn + d.dtc2 * gt[n][1];
+                        if (d.tstep >= 3) chn = chn + d.dtc3 * gg;
+                        chn = chn / fld_r(a.timpl[n], ro, gi, j, oe);
+                        stg(tb + (1 - d.cur) * pl, gi, chn);
+                        stg(tb + (2 + r2) * pl, gi, gg);
+                    }
+                }
+            }
+        }
+    };
+    // rows j0-1 .. jend, unrolled by three so the roles rotate without copies
+    MRow<T, NTR> A, B, C;
+    for (int k = j0 - 1; k <= jend; k += 3) {
+        row(k, A, C, B);
+        if (k + 1 <= jend) row(k + 1, B, A, C);
+        if (k + 2 <= jend) row(k + 2, C, B, A);
+    }
+}
+
 #undef OE
 #undef OV
 #undef OU
@@ -589,6 +983,95 @@ divergence_kernel(int Nx, int Ny, int P, const T* u, const T* v, const uint8_t* 
         zb = bc * (cv[jp] * v[(size_t)jp * P + i] - cv[j] * v[off]) / AdT;
     }
     div[off] = za + zb / ce[j];
+}
+
+// ---------------------------------------------------------------------------
+// Conservation sums in FP64 with compensated (Neumaier) accumulation, for the
+// FP32 storage path as well as the FP64 one.  Over ocean eta points, weighted
+// by cos(lat_eta):
+//   [0] volume  sum eta
+//   [1] energy  sum 0.5*D*(0.5*(u(i)^2 + u(i+1)^2) + 0.5*(v(j)^2 + v(j+1)^2)) + 0.5*G*eta^2
+//   [2] tracer  sum CH
+// Cells are visited in a fixed order per thread and combined in a fixed tree,
+// so the result is reproducible run to run for a given grid.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void nsum_add(double& s, double& c, double x) {
+    const double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+sums_kernel(const Statics<T>* sp, int fn, const T* eta, const T* u, const T* v, const T* ch, const double* ce,
+            double* partial) {
+    const Statics<T>& a = *sp;
+    const int Nx = a.Nx, Ny = a.Ny, P = a.P;
+    double s[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
+    const long n = (long)Nx * Ny;
+    for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (long)gridDim.x * 256) {
+        const int j = (int)(idx / Nx), i = (int)(idx - (long)j * Nx);
+        const size_t off = (size_t)j * P + i;
+        if (!(a.mask[off] & OC_E)) continue;
+        const int ip = i + 1 == Nx ? 0 : i + 1, jp = j + 1 == Ny ? 0 : j + 1;
+        const double w = ce[j];
+        const double e = (double)eta[off];
+        const double H = (double)fld(a.He, off, j, true);
+        const double D = fn ? e + H : H;
+        const double uc = (double)u[off], ue = (double)u[(size_t)j * P + ip];
+        const double vc = (double)v[off], vn = (double)v[(size_t)jp * P + i];
+        const double ke = 0.5 * D * (0.5 * (uc * uc + ue * ue) + 0.5 * (vc * vc + vn * vn));
+        const double pe = 0.5 * (double)a.G * (e * e);
+        nsum_add(s[0], c[0], e * w);
+        nsum_add(s[1], c[1], (ke + pe) * w);
+        if (ch) nsum_add(s[2], c[2], (double)ch[off] * w);
+    }
+    __shared__ double rs[3][256], rc[3][256];
+    const int tid = threadIdx.x;
+    for (int k = 0; k < 3; ++k) { rs[k][tid] = s[k]; rc[k][tid] = c[k]; }
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if (tid < stride)
+            for (int k = 0; k < 3; ++k) {
+                double ss = rs[k][tid], cc = rc[k][tid] + rc[k][tid + stride];
+                nsum_add(ss, cc, rs[k][tid + stride]);
+                rs[k][tid] = ss;
+                rc[k][tid] = cc;
+            }
+        __syncthreads();
+    }
+    if (tid == 0)
+        for (int k = 0; k < 3; ++k) {
+            partial[(size_t)blockIdx.x * 6 + 2 * k] = rs[k][0];
+            partial[(size_t)blockIdx.x * 6 + 2 * k + 1] = rc[k][0];
+        }
+}
+
+__global__ void __launch_bounds__(256)
+sums_final(const double* partial, int nblk, double* out) {
+    double s[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
+    const int tid = threadIdx.x;
+    for (int b = tid; b < nblk; b += 256)
+        for (int k = 0; k < 3; ++k) {
+            c[k] += partial[(size_t)b * 6 + 2 * k + 1];
+            nsum_add(s[k], c[k], partial[(size_t)b * 6 + 2 * k]);
+        }
+    __shared__ double rs[3][256], rc[3][256];
+    for (int k = 0; k < 3; ++k) { rs[k][tid] = s[k]; rc[k][tid] = c[k]; }
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if (tid < stride)
+            for (int k = 0; k < 3; ++k) {
+                double ss = rs[k][tid], cc = rc[k][tid] + rc[k][tid + stride];
+                nsum_add(ss, cc, rs[k][tid + stride]);
+                rs[k][tid] = ss;
+                rc[k][tid] = cc;
+            }
+        __syncthreads();
+    }
+    if (tid == 0)
+        for (int k = 0; k < 3; ++k) out[k] = rs[k][0] + rc[k][0];
 }
 
 }  // namespace swm

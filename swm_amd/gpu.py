@@ -29,8 +29,8 @@ FP64, FP32 = 0, 1
 
 EXPORTED = [
     "swm_gpu_create", "swm_gpu_upload_state", "swm_gpu_add_tracer", "swm_gpu_step", "swm_gpu_itt",
-    "swm_gpu_sync", "swm_gpu_sor", "swm_gpu_sor_fixed", "swm_gpu_divergence", "swm_gpu_last_timing",
-    "swm_gpu_last_error", "swm_gpu_destroy",
+    "swm_gpu_sync", "swm_gpu_sor", "swm_gpu_sor_fixed", "swm_gpu_divergence", "swm_gpu_sums",
+    "swm_gpu_last_timing", "swm_gpu_last_error", "swm_gpu_destroy",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -95,6 +95,8 @@ def load_library(path: str = LIB_PATH):
     L.swm_gpu_sor_fixed.restype = C.c_int
     L.swm_gpu_divergence.argtypes = [p, _dp]
     L.swm_gpu_divergence.restype = C.c_int
+    L.swm_gpu_sums.argtypes = [p, C.c_int32, _dp]
+    L.swm_gpu_sums.restype = C.c_int
     L.swm_gpu_last_timing.argtypes = [p, _dp, _dp]
     L.swm_gpu_last_timing.restype = C.c_int
     L.swm_gpu_last_error.argtypes = [p]
@@ -264,6 +266,13 @@ class SwmGpu:
         out = np.empty((self.s.Ny, self.s.Nx), dtype=np.float64)
         self._check(self.L.swm_gpu_divergence(self._h, out.ctypes.data_as(_dp)), "swm_gpu_divergence")
         return out
+
+    def sums(self, tracer: int = -1):
+        """FP64 compensated (volume, energy, tracer inventory) sums, cos-weighted
+        over ocean eta points (swm_gpu_sums)."""
+        out = (C.c_double * 3)()
+        self._check(self.L.swm_gpu_sums(self._h, int(tracer), C.cast(out, _dp)), "swm_gpu_sums")
+        return float(out[0]), float(out[1]), float(out[2])
 
     # -- elliptic solver ------------------------------------------------------
     def sor(self, B, chi, epsilon=1e-4, first_guess=True):
