@@ -93,6 +93,81 @@ def cpu_baseline(s, budget_s: float):
                       f"OpenMP schedule(static,5) over rows)"}
 
 
+def plain_fp64_rate(steps: int = 1000) -> float:
+    """cell-updates/s of the plain FP64 path on the 1024 x 1024 workload (dt = 30 s)."""
+    from swm_amd import configs, gpu
+    s = configs.c2_nonlinear(seed=0, N=1024, dt=30.0)
+    g = gpu.SwmGpu(s).init_from_setup()
+    g.step(50)
+    g.step(steps)
+    total_ms, _ = g.last_timing()
+    g.close()
+    return s.Nx * s.Ny * steps / (total_ms * 1e-3)
+
+
+def run_ensemble(setups, precision, warmup, steps):
+    """Create, attach and advance every member; returns (members, device ms of the
+    timed steps)."""
+    from swm_amd import gpu
+    ms = [gpu.SwmGpu(s, precision).init_from_setup() for s in setups]
+    b = gpu.SwmBatch()
+    for m in ms:
+        b.attach(m, warmup + steps)
+    b.step(warmup)
+    b.step(steps)
+    t = b.last_ms()
+    b.close()
+    return ms, t
+
+
+def bench_ensemble(args):
+    """Batched ensemble: 64 uneven members, one launch per step, FP64 and FP32."""
+    import numpy as np
+    from swm_amd import configs, gpu
+    t0 = time.perf_counter()
+    setups = configs.c4_ensemble(seed=0, n=args.members, nsteps=args.warmup + args.steps)
+    t_init = time.perf_counter() - t0
+    cells = [s.Nx * s.Ny for s in setups]
+    ntr = [len(s.tracers) for s in setups]
+    bytes_f64 = sum(c * (121 + 40 * t) for c, t in zip(cells, ntr))
+    bytes_f32 = sum(c * (61 + 20 * t) for c, t in zip(cells, ntr))
+    m64, ms64 = run_ensemble(setups, gpu.FP64, args.warmup, args.steps)
+    m32, ms32 = run_ensemble(setups, gpu.FP32, args.warmup, args.steps)
+    err, drift = [], []
+    for s, a, b in zip(setups, m64, m32):
+        oc = s.grids["eta"].ocean == 1
+        e64, e32 = a.get("SWM_ETA")[oc], b.get("SWM_ETA")[oc]
+        err.append(float(np.abs(e32 - e64).max() / max(np.abs(e64).max(), 1e-300)))
+        w = np.broadcast_to(s.grids["eta"].cos_lat[:, None], s.shape)[oc]
+        v64, v32 = a.sums()[0], b.sums()[0]
+        drift.append(abs(v32 - v64) / max(float((np.abs(e64) * w).sum()), 1e-300))
+        a.close()
+        b.close()
+    total = sum(cells) * args.steps
+    v64, v32 = total / (ms64 * 1e-3), total / (ms32 * 1e-3)
+    plain = plain_fp64_rate()
+    out = {
+        "metric": METRIC, "value": v32, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms32 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batched ensemble, {len(setups)} nonlinear basins of uneven size "
+                               f"(nx, ny in {list(configs.ENSEMBLE_SIZES)}), H~U(200,4000), Ah~U(20,500), "
+                               f"tau~U(0.02,0.2), {sum(1 for t in ntr if t)} with a tracer",
+                   "members": len(setups), "cells": sum(cells), "host_init_s": t_init},
+        "roofline": {"bound": "hbm", "achieved": bytes_f32 / (ms32 * 1e-3 / args.steps) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": bytes_f32 / (ms32 * 1e-3 / args.steps) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_step": bytes_f32},
+        "fp64": {"value": v64, "ms_per_step": ms64 / args.steps,
+                 "achieved_GBs": bytes_f64 / (ms64 * 1e-3 / args.steps) / 1e9},
+        "plain_fp64_c2_value": plain,
+        "fp32_vs_plain_fp64": v32 / plain,
+        "fp32_error_vs_fp64": {"eta_rel_maxnorm_max": max(err), "eta_rel_maxnorm_median": float(np.median(err)),
+                               "volume_drift_rel_max": max(drift)},
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +178,9 @@ def main(argv=None):
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["c2", "ensemble"], default="c2",
+                    help="c2: the 1024 x 1024 basin (default); ensemble: the batched 64-member ensemble")
+    ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy four times (counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -120,6 +198,8 @@ def main(argv=None):
         gbs, sec = copy_bandwidth(256 << 20)
         print(json.dumps({"copy_bytes_each_way": 256 << 20, "copies": 5, "GB/s": gbs, "s_per_copy": sec}))
         return 0
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
     from swm_amd import configs, gpu
 
     s = configs.c2_nonlinear(seed=0, N=args.n, dt=args.dt, nsteps=args.warmup + args.steps)

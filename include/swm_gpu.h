@@ -163,6 +163,35 @@ int swm_gpu_divergence(swm_ctx *c, double *host_div);
  * A^2*dLambda*dTheta for volumes.  Blocks until the result is on the host. */
 int swm_gpu_sums(swm_ctx *c, int32_t tracer_id, double *out);
 
+/* Batched ensembles.  No counterpart in the reference, which advances one
+ * domain per process (model.f90:159-196 per member); these serve callers that
+ * run many independent domains of uneven size at once (ensembles, a request
+ * server).  Members are ordinary contexts, all of one precision and variant,
+ * without oscillating forcing.  Each batch step is one launch covering every
+ * tile of every attached member (SWM_timestep + TRC_timestep + model_advance
+ * per member, the same arithmetic as swm_gpu_step, so a member's result is the
+ * same alone and in the batch) plus a launch that moves the members' step
+ * state on; the pair is captured once in a hipGraph and replayed.
+ *   attach:    the context joins and will advance nsteps more steps; while
+ *              attached, swm_gpu_step / upload_state / add_tracer on it fail,
+ *              and sync / sums / divergence / itt see its current state.
+ *   step:      advances every member with steps left by up to nsteps steps
+ *              (members stop at their own count); *active (optional) = members
+ *              with steps left afterwards.
+ *   detach:    the context leaves with its state; set_steps resets a member's
+ *              remaining count; remaining reads it. */
+typedef struct swm_batch swm_batch;
+int swm_gpu_batch_create(swm_batch **out);
+int swm_gpu_batch_attach(swm_batch *b, swm_ctx *c, int64_t nsteps);
+int swm_gpu_batch_detach(swm_batch *b, swm_ctx *c);
+int swm_gpu_batch_set_steps(swm_batch *b, swm_ctx *c, int64_t nsteps);
+int swm_gpu_batch_remaining(swm_batch *b, const swm_ctx *c, int64_t *remaining);
+int swm_gpu_batch_step(swm_batch *b, int64_t nsteps, int32_t *active);
+/* device time of the last swm_gpu_batch_step (ms) */
+int swm_gpu_batch_last_timing(const swm_batch *b, double *total_ms);
+const char *swm_gpu_batch_last_error(const swm_batch *b);
+void swm_gpu_batch_destroy(swm_batch *b);
+
 /* Time spent in the last swm_gpu_step call's kernels (ms, device clock) and
  * the average duration of the step kernel (ms), for roofline accounting. */
 int swm_gpu_last_timing(const swm_ctx *c, double *total_ms, double *step_kernel_ms);

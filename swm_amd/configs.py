@@ -128,3 +128,43 @@ def c3_elliptic(seed=0, Nx=2048, Ny=1024, dx=10e3, dt=30.0, nsteps=2000, Ah=200.
     H.add_windstress(s, H.COMP_X, double_gyre_tau(s, tau0))
     s.Nt = nsteps
     return H.finalize(s)
+
+
+ENSEMBLE_SIZES = (128, 256, 384, 512, 768, 1024)
+
+
+def stable_dt(H0: float, dx: float, G: float = 9.81, cfl: float = 0.2) -> float:
+    """Time step with gravity-wave Courant number c*dt/dx <= cfl (c = sqrt(G*H0)),
+    rounded down to whole seconds: AB3 on the C-grid stays stable there."""
+    return float(max(1.0, math.floor(cfl * dx / math.sqrt(G * H0))))
+
+
+def ensemble_member(seed: int, Nx: int, Ny: int, H0: float, Ah: float, tau0: float, tracer: bool,
+                    dx: float = 5e3, nsteps: int = 5000):
+    """One nonlinear beta-plane basin of the batched ensemble: the C2 physics with
+    the member's depth, viscosity and wind amplitude, dt from stable_dt."""
+    s = basin(Nx, Ny, dx, dx, variant=H.FULLY_NONLINEAR, H0=H0, dt=stable_dt(H0, dx), G=9.81, Ah=Ah, lbc=2.0,
+              f0=1e-4, beta=2e-11, drag=1e-6, lateral_mixing=True, land_fraction=0.05, seed=seed)
+    H.add_windstress(s, H.COMP_X, double_gyre_tau(s, tau0))
+    if tracer:
+        H.add_tracer(s, fourier_modes(s, s._rng), kappa_h=50.0)
+    s.Nt = nsteps
+    return H.finalize(s)
+
+
+def ensemble_params(seed: int = 0, n: int = 64):
+    """Parameters of the batched ensemble: nx, ny each from ENSEMBLE_SIZES,
+    H ~ U(200, 4000) m, Ah ~ U(20, 500) m^2/s, wind amplitude ~ U(0.02, 0.2) N/m^2,
+    a tracer with probability 1/2."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        out.append(dict(seed=seed * 100003 + k, Nx=int(rng.choice(ENSEMBLE_SIZES)), Ny=int(rng.choice(ENSEMBLE_SIZES)),
+                        H0=float(rng.uniform(200.0, 4000.0)), Ah=float(rng.uniform(20.0, 500.0)),
+                        tau0=float(rng.uniform(0.02, 0.2)), tracer=bool(rng.random() < 0.5)))
+    return out
+
+
+def c4_ensemble(seed: int = 0, n: int = 64, nsteps: int = 5000):
+    """The batched ensemble: n independent members of uneven size."""
+    return [ensemble_member(nsteps=nsteps, **p) for p in ensemble_params(seed, n)]

@@ -107,6 +107,7 @@ struct swm_ctx {
     void* sor_buf[2][2] = {};    // [fp32][chi|B]
     void* sor_ftab[2][4] = {};   // coefficient tables in the sweep precision
     void* dza = nullptr;
+    swm_batch* batch = nullptr;  // batch this context is attached to
 };
 
 namespace {
@@ -587,6 +588,8 @@ int alloc_state(swm_ctx* c, int ntrc) {
     return 0;
 }
 
+int member_refresh(swm_ctx* c);
+
 }  // namespace
 
 extern "C" {
@@ -703,6 +706,7 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
 int swm_gpu_upload_state(swm_ctx* c, const double* u, const double* v, const double* eta, const double* G_u,
                          const double* G_v, const double* G_eta, int64_t itt) {
     if (!c) return 1;
+    if (c->batch) return fail(c, "context is attached to a batch; detach it first", 1);
     if (!u || !v || !eta) return fail(c, "u, v and eta are required", 1);
     if (itt < 0) return fail(c, "itt must be >= 0", 1);
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -726,6 +730,7 @@ int swm_gpu_upload_state(swm_ctx* c, const double* u, const double* v, const dou
 
 int swm_gpu_add_tracer(swm_ctx* c, const swm_tracer_desc* t, int32_t* tracer_id) {
     if (!c || !t) return 1;
+    if (c->batch) return fail(c, "context is attached to a batch; detach it first", 1);
     if ((int)c->trc.size() >= kMaxTracers) return fail(c, "at most 2 tracers are fused into the step kernel", 4);
     if (!t->CH || !t->C) return fail(c, "tracer CH and C are required", 1);
     if (c->d.variant != SWM_FULLY_NONLINEAR)
@@ -760,6 +765,7 @@ int swm_gpu_add_tracer(swm_ctx* c, const swm_tracer_desc* t, int32_t* tracer_id)
 
 int swm_gpu_step(swm_ctx* c, int64_t nsteps) {
     if (!c) return 1;
+    if (c->batch) return fail(c, "context is attached to a batch; it advances through swm_gpu_batch_step", 1);
     if (nsteps < 0) return fail(c, "nsteps must be >= 0", 1);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int rc = c->fp32 ? step_many<float>(c, nsteps) : step_many<double>(c, nsteps);
@@ -773,7 +779,11 @@ int swm_gpu_step(swm_ctx* c, int64_t nsteps) {
     return 0;
 }
 
-int64_t swm_gpu_itt(const swm_ctx* c) { return c ? c->itt : -1; }
+int64_t swm_gpu_itt(const swm_ctx* c) {
+    if (!c) return -1;
+    if (c->batch && member_refresh(const_cast<swm_ctx*>(c))) return -1;
+    return c->itt;
+}
 
 int swm_gpu_last_timing(const swm_ctx* c, double* total_ms, double* step_kernel_ms) {
     if (!c) return 1;
@@ -784,6 +794,7 @@ int swm_gpu_last_timing(const swm_ctx* c, double* total_ms, double* step_kernel_
 
 int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_out) {
     if (!c || !name || !host_out) return 1;
+    if (int rc = member_refresh(c)) return rc;
     std::string s(name);
     for (auto& ch : s) ch = (char)toupper((unsigned char)ch);
     const int n0 = c->cur;
@@ -986,6 +997,7 @@ int swm_gpu_sor_fixed(swm_ctx* c, const double* B, double* chi, double omega, do
 
 int swm_gpu_divergence(swm_ctx* c, double* host_div) {
     if (!c || !host_div) return 1;
+    if (int rc = member_refresh(c)) return rc;
     const int n = c->Nx * c->Ny;
     const int nb = (n + 255) / 256;
     if (c->fp32)
@@ -1007,6 +1019,7 @@ int swm_gpu_divergence(swm_ctx* c, double* host_div) {
 
 int swm_gpu_sums(swm_ctx* c, int32_t tracer_id, double* out) {
     if (!c || !out) return 1;
+    if (int rc = member_refresh(c)) return rc;
     if (tracer_id >= (int32_t)c->trc.size()) return fail(c, "tracer id out of range", 1);
     const long n = (long)c->Nx * c->Ny;
     const int nb = (int)std::min<long>(1024, (n + 255) / 256);
@@ -1046,6 +1059,7 @@ const char* swm_gpu_last_error(const swm_ctx* c) { return c ? c->err.c_str() : g
 
 void swm_gpu_destroy(swm_ctx* c) {
     if (!c) return;
+    if (c->batch) (void)swm_gpu_batch_detach(c->batch, c);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graphs(c);
     for (void* p : c->allocs) (void)hipFree(p);
@@ -1053,6 +1067,305 @@ void swm_gpu_destroy(swm_ctx* c) {
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched ensembles: contexts attached to one batch advance together, one
+// launch (plus a tiny state-advance launch) per step, replayed from a graph.
+// ---------------------------------------------------------------------------
+struct swm_batch {
+    int fp32 = -1;                       // precision of the members (-1: none attached yet)
+    int variant = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<swm_ctx*> mem;
+    std::vector<swm::MemberStep> hstep;  // host copy of the members' step state
+    bool on_device = false;              // the device copy is newer than hstep
+    bool dirty = true;                   // member list changed since the last build
+    void* dmem = nullptr;                // Member<T>[cap]
+    swm::MemberStep* dstep = nullptr;    // MemberStep[cap]
+    size_t cap = 0;
+    int* downer = nullptr;               // member of each work item
+    size_t owner_cap = 0;
+    int nwork = 0, ntrc_max = 0;
+    hipGraphExec_t graph = nullptr;
+    double last_ms = 0.0;
+    std::string err;
+};
+
+namespace {
+
+constexpr int kBatchTx = 64, kBatchTy = 4;
+
+int bfail(swm_batch* b, const std::string& msg, int code = 2) {
+    if (b) b->err = msg;
+    return code;
+}
+
+#define BHIPCHK(b, call)                                                                       \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return bfail((b), std::string(#call) + ": " + hipGetErrorString(e_), 3);           \
+    } while (0)
+
+void batch_drop_graph(swm_batch* b) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+}
+
+// Copy the device step state back and hand it to the member contexts.
+int batch_pull(swm_batch* b) {
+    if (!b->on_device) return 0;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    if (!b->hstep.empty())
+        BHIPCHK(b, hipMemcpy(b->hstep.data(), b->dstep, b->hstep.size() * sizeof(swm::MemberStep),
+                             hipMemcpyDeviceToHost));
+    b->on_device = false;
+    for (size_t k = 0; k < b->mem.size(); ++k) {
+        swm_ctx* c = b->mem[k];
+        const swm::MemberStep& h = b->hstep[k];
+        c->itt = h.itt;
+        c->cur = h.cur;
+        c->ring = h.ring;
+        for (size_t n = 0; n < c->trc.size(); ++n) c->trc[n].cbuf_valid = ((h.cbuf >> n) & 1) != 0;
+    }
+    return 0;
+}
+
+template <typename T>
+int batch_build_t(swm_batch* b) {
+    const size_t n = b->mem.size();
+    if (n > b->cap) {
+        if (b->dmem) (void)hipFree(b->dmem);
+        if (b->dstep) (void)hipFree(b->dstep);
+        b->dmem = nullptr;
+        b->dstep = nullptr;
+        b->cap = 0;
+        const size_t cap = std::max<size_t>(n, 2 * b->cap + 16);
+        BHIPCHK(b, hipMalloc(&b->dmem, cap * sizeof(swm::Member<T>)));
+        BHIPCHK(b, hipMalloc((void**)&b->dstep, cap * sizeof(swm::MemberStep)));
+        b->cap = cap;
+    }
+    std::vector<swm::Member<T>> hm(n);
+    long work = 0;
+    int ntrc_max = 0;
+    for (size_t k = 0; k < n; ++k) {
+        swm_ctx* c = b->mem[k];
+        swm::Member<T>& m = hm[k];
+        m.s = (const Statics<T>*)c->statics;
+        m.state = (T*)c->state;
+        m.plane = c->plane;
+        m.ntx = (c->Nx + kBatchTx - 1) / kBatchTx;
+        m.ntile = m.ntx * ((c->Ny + kBatchTy - 1) / kBatchTy);
+        m.wbase = (int)work;
+        m.ntr = (int)c->trc.size();
+        for (int t = 0; t < 3; ++t)
+            for (int q = 0; q < 3; ++q) m.dtc[t][q] = (T)(c->d.dt * c->ab[t][q]);
+        work += m.ntile;
+        ntrc_max = std::max(ntrc_max, m.ntr);
+    }
+    if (work > INT32_MAX) return bfail(b, "batch too large for one launch", 4);
+    b->nwork = (int)work;
+    b->ntrc_max = ntrc_max;
+    if ((size_t)work > b->owner_cap) {
+        if (b->downer) (void)hipFree(b->downer);
+        b->downer = nullptr;
+        b->owner_cap = 0;
+        BHIPCHK(b, hipMalloc((void**)&b->downer, (size_t)work * sizeof(int)));
+        b->owner_cap = (size_t)work;
+    }
+    std::vector<int> owner((size_t)work);
+    for (size_t k = 0; k < n; ++k)
+        std::fill(owner.begin() + hm[k].wbase, owner.begin() + hm[k].wbase + hm[k].ntile, (int)k);
+    if (work)
+        BHIPCHK(b, hipMemcpyAsync(b->downer, owner.data(), (size_t)work * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    if (n) {
+        BHIPCHK(b, hipMemcpyAsync(b->dmem, hm.data(), n * sizeof(swm::Member<T>), hipMemcpyHostToDevice, b->stream));
+        BHIPCHK(b, hipMemcpyAsync(b->dstep, b->hstep.data(), n * sizeof(swm::MemberStep), hipMemcpyHostToDevice,
+                                  b->stream));
+        BHIPCHK(b, hipStreamSynchronize(b->stream));
+    }
+    b->on_device = n > 0;
+    b->dirty = false;
+    batch_drop_graph(b);
+    return 0;
+}
+
+template <typename T, int V>
+void batch_launch_v(swm_batch* b) {
+    const int nmem = (int)b->mem.size();
+    dim3 grid(b->nwork), block(256);
+    const swm::Member<T>* m = (const swm::Member<T>*)b->dmem;
+    switch (b->ntrc_max) {
+    case 0: hipLaunchKernelGGL((swm::batch_kernel<T, V, 0, kBatchTx, kBatchTy>), grid, block, 0, b->stream, m, b->dstep, b->downer, b->nwork); break;
+    case 1: hipLaunchKernelGGL((swm::batch_kernel<T, V, 1, kBatchTx, kBatchTy>), grid, block, 0, b->stream, m, b->dstep, b->downer, b->nwork); break;
+    default: hipLaunchKernelGGL((swm::batch_kernel<T, V, 2, kBatchTx, kBatchTy>), grid, block, 0, b->stream, m, b->dstep, b->downer, b->nwork); break;
+    }
+    hipLaunchKernelGGL(swm::batch_advance, dim3((nmem + 255) / 256), dim3(256), 0, b->stream, b->dstep, nmem);
+}
+
+void batch_launch(swm_batch* b) {
+    if (b->fp32) {
+        if (b->variant == SWM_FULLY_NONLINEAR) batch_launch_v<float, swm::VAR_FN>(b);
+        else batch_launch_v<float, swm::VAR_SOR>(b);
+    } else {
+        if (b->variant == SWM_FULLY_NONLINEAR) batch_launch_v<double, swm::VAR_FN>(b);
+        else batch_launch_v<double, swm::VAR_SOR>(b);
+    }
+}
+
+int member_refresh(swm_ctx* c) {
+    if (!c->batch) return 0;
+    if (batch_pull(c->batch)) return fail(c, "batch: " + c->batch->err, 3);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int swm_gpu_batch_create(swm_batch** out) {
+    if (!out) return 1;
+    *out = nullptr;
+    swm_batch* b = new swm_batch();
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        g_create_err = "batch: could not create a HIP stream/events";
+        swm_gpu_batch_destroy(b);
+        return 3;
+    }
+    *out = b;
+    return 0;
+}
+
+int swm_gpu_batch_attach(swm_batch* b, swm_ctx* c, int64_t nsteps) {
+    if (!b || !c) return 1;
+    if (c->batch) return bfail(b, "context is already attached to a batch", 1);
+    if (nsteps < 0) return bfail(b, "nsteps must be >= 0", 1);
+    if (c->has_osc) return bfail(b, "oscillating forcing needs per-step host phases; step such a context alone", 4);
+    if (b->fp32 >= 0 && (b->fp32 != (int)c->fp32 || b->variant != c->d.variant))
+        return bfail(b, "every member of a batch has the same precision and variant", 4);
+    if (int rc = batch_pull(b)) return rc;
+    BHIPCHK(b, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    b->fp32 = c->fp32;
+    b->variant = c->d.variant;
+    swm::MemberStep h{};
+    h.itt = c->itt;
+    h.remaining = nsteps;
+    h.cur = c->cur;
+    h.ring = c->ring;
+    for (size_t n = 0; n < c->trc.size(); ++n)
+        if (c->trc[n].cbuf_valid) h.cbuf |= 1 << n;
+    b->mem.push_back(c);
+    b->hstep.push_back(h);
+    c->batch = b;
+    b->dirty = true;
+    return 0;
+}
+
+int swm_gpu_batch_detach(swm_batch* b, swm_ctx* c) {
+    if (!b || !c) return 1;
+    if (c->batch != b) return bfail(b, "context is not attached to this batch", 1);
+    if (int rc = batch_pull(b)) return rc;
+    const size_t k = (size_t)(std::find(b->mem.begin(), b->mem.end(), c) - b->mem.begin());
+    b->mem.erase(b->mem.begin() + k);
+    b->hstep.erase(b->hstep.begin() + k);
+    c->batch = nullptr;
+    b->dirty = true;
+    if (b->mem.empty()) b->fp32 = -1;
+    return 0;
+}
+
+int swm_gpu_batch_set_steps(swm_batch* b, swm_ctx* c, int64_t nsteps) {
+    if (!b || !c) return 1;
+    if (c->batch != b) return bfail(b, "context is not attached to this batch", 1);
+    if (nsteps < 0) return bfail(b, "nsteps must be >= 0", 1);
+    if (int rc = batch_pull(b)) return rc;
+    const size_t k = (size_t)(std::find(b->mem.begin(), b->mem.end(), c) - b->mem.begin());
+    b->hstep[k].remaining = nsteps;
+    b->dirty = true;
+    return 0;
+}
+
+int swm_gpu_batch_remaining(swm_batch* b, const swm_ctx* c, int64_t* remaining) {
+    if (!b || !c || !remaining) return 1;
+    if (c->batch != b) return bfail(b, "context is not attached to this batch", 1);
+    if (int rc = batch_pull(b)) return rc;
+    const size_t k = (size_t)(std::find(b->mem.begin(), b->mem.end(), c) - b->mem.begin());
+    *remaining = b->hstep[k].remaining;
+    return 0;
+}
+
+int swm_gpu_batch_step(swm_batch* b, int64_t nsteps, int32_t* active) {
+    if (!b) return 1;
+    if (nsteps < 0) return bfail(b, "nsteps must be >= 0", 1);
+    if (b->dirty) {
+        if (int rc = batch_pull(b)) return rc;
+        int rc = b->fp32 == 1 ? batch_build_t<float>(b) : batch_build_t<double>(b);
+        if (rc) return rc;
+    }
+    b->last_ms = 0.0;
+    if (!b->mem.empty() && nsteps > 0) {
+        if (!b->graph) {
+            hipGraph_t g = nullptr;
+            BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            batch_launch(b);
+            const hipError_t le = hipGetLastError();
+            const hipError_t ce = hipStreamEndCapture(b->stream, &g);
+            if (le != hipSuccess || ce != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return bfail(b, std::string("batch graph capture: ") + hipGetErrorString(le != hipSuccess ? le : ce), 3);
+            }
+            const hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                b->graph = nullptr;
+                return bfail(b, std::string("batch graph instantiate: ") + hipGetErrorString(ie), 3);
+            }
+        }
+        BHIPCHK(b, hipEventRecord(b->ev0, b->stream));
+        for (int64_t k = 0; k < nsteps; ++k) BHIPCHK(b, hipGraphLaunch(b->graph, b->stream));
+        BHIPCHK(b, hipEventRecord(b->ev1, b->stream));
+        BHIPCHK(b, hipEventSynchronize(b->ev1));
+        float ms = 0.f;
+        BHIPCHK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->last_ms = ms;
+        b->on_device = true;
+    }
+    if (active) {
+        if (int rc = batch_pull(b)) return rc;
+        int32_t na = 0;
+        for (const auto& h : b->hstep) na += h.remaining > 0;
+        *active = na;
+    }
+    return 0;
+}
+
+int swm_gpu_batch_last_timing(const swm_batch* b, double* total_ms) {
+    if (!b || !total_ms) return 1;
+    *total_ms = b->last_ms;
+    return 0;
+}
+
+const char* swm_gpu_batch_last_error(const swm_batch* b) { return b ? b->err.c_str() : g_create_err.c_str(); }
+
+void swm_gpu_batch_destroy(swm_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    (void)batch_pull(b);
+    for (swm_ctx* c : b->mem) c->batch = nullptr;
+    batch_drop_graph(b);
+    if (b->dmem) (void)hipFree(b->dmem);
+    if (b->dstep) (void)hipFree(b->dstep);
+    if (b->downer) (void)hipFree(b->downer);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
 }
 
 }  // extern "C"

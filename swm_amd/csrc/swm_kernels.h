@@ -120,6 +120,16 @@ __device__ __forceinline__ T interp_noland2(T D1, T D2, int m1, int m2) {
     return (m1 ? w : T(0)) * D1 + (m2 ? w : T(0)) * D2;
 }
 
+// Arithmetic of the step kernels.  FP64 evaluates every division and
+// multiply-add exactly as written (the file is compiled without contraction),
+// which keeps FP64 bit-identical to the reference order.  FP32 storage has a
+// stated error bar instead (1e-4 relative max-norm on eta against FP64), so it
+// divides through the hardware reciprocal and fuses multiply-adds.
+__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+__device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double madd(double acc, double x, double y) { return acc + x * y; }
+__device__ __forceinline__ float madd(float acc, float x, float y) { return __builtin_fmaf(x, y, acc); }
+
 // Tracer advection coefficient TRC_coeff(k) (src/tracer_module.f90:318-333):
 // sign * num_mask / den / A / d / cos, den = ocean_eta(nb) + ocean_eta(i,j).
 // den == 0 is the reference's 0/0 (NaN); a zero numerator gives a signed zero.
@@ -131,22 +141,76 @@ __device__ __forceinline__ T trc_adv(bool neg, int nummask, int den, const T* ta
     return neg ? -v : v;
 }
 
-template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG>
-__global__ void __launch_bounds__(256)
-step_kernel(const Dyn<T> d) {
+// Loads of the step kernels: per-lane values through a global pointer with
+// a 32-bit lane index (base address stays in SGPRs), and uniform values of the
+// read-only per-row tables through the constant address space (scalar loads).
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p, unsigned i) {
+    return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void stg(T* p, unsigned i, T v) {
+    ((__attribute__((address_space(1))) T*)p)[i] = v;
+}
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p, size_t i) {
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+// fld() for a uniform row: ro = row offset, j = row index
+template <typename T>
+__device__ __forceinline__ T fld_r(const Fld<T>& f, size_t ro, unsigned gi, int j, bool ocean) {
+    if (f.full) return ldg(f.full + ro, gi);
+    const T o = ldc(f.rows, 2 * (size_t)j), l = ldc(f.rows, 2 * (size_t)j + 1);
+    return ocean ? o : l;
+}
+// trc_adv() for a uniform row
+template <typename T>
+__device__ __forceinline__ T trc_adv_r(bool neg, int nummask, int den, const T* tab, int j) {
+    const T v1 = ldc(tab, 2 * (size_t)j), v2 = ldc(tab, 2 * (size_t)j + 1);
+    if (den == 0) return __builtin_nan("");
+    if (!nummask) return neg ? T(-0.0) : T(0.0);
+    const T v = den == 1 ? v1 : v2;
+    return neg ? -v : v;
+}
+
+// LDS of one tile: the state with a one-cell halo, the derived fields and the
+// per-row coefficient tables of its rows.  NTR is the largest tracer count the
+// kernel handles.
+template <typename T, int NTR, int E, int EH>
+struct TileLds {
+    T u[E], v[E], eta[E], D[E], Pot[E], Ed[E], MU[E], MV[E], pll[E], plt[E];
+    T CH[NTR][E], C[NTR][E];
+    // per-row tables of the EH tile rows (cu also of the row above them)
+    T za[EH][2], cu[EH + 1], cH[EH], fH[EH], pltc[EH][12], pllc[EH][12];
+    uint8_t m[E];
+};
+
+// One 256-thread block advances tile t (TX x TY cells) of the domain in d by
+// one step.  L is a TileLds with room for at least NTRC tracers.
+template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, typename L>
+__device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) {
     constexpr int NT = 256;
     constexpr int EW = TX + 2;
     constexpr int EH = TY + 2;
     constexpr int E = EW * EH;
     constexpr int NO = (TX * TY + NT - 1) / NT;
     constexpr int NTR = NTRC > 0 ? NTRC : 1;
-    __shared__ T s_u[E], s_v[E], s_eta[E], s_D[E], s_Pot[E], s_Ed[E], s_MU[E], s_MV[E], s_pll[E], s_plt[E];
-    __shared__ T s_CH[NTR][E], s_C[NTR][E];
-    __shared__ uint8_t s_m[E];
+    T* const s_u = lds.u;
+    T* const s_v = lds.v;
+    T* const s_eta = lds.eta;
+    T* const s_D = lds.D;
+    T* const s_Pot = lds.Pot;
+    T* const s_Ed = lds.Ed;
+    T* const s_MU = lds.MU;
+    T* const s_MV = lds.MV;
+    T* const s_pll = lds.pll;
+    T* const s_plt = lds.plt;
+    auto& s_CH = lds.CH;
+    auto& s_C = lds.C;
+    uint8_t* const s_m = lds.m;
 
     const Statics<T>& a = *d.s;
     const int tid = threadIdx.x;
-    const int t = xcd_tile(blockIdx.x, d.nblk);
     const int i0 = (t % d.ntx) * TX;
     const int j0 = (t / d.ntx) * TY;
     const int Nx = a.Nx, Ny = a.Ny, P = a.P;
@@ -188,31 +252,94 @@ step_kernel(const Dyn<T> d) {
         }
     }
 
-    // ---- phase 1: state with a one-cell halo (periodic indices) into LDS
-    for (int e = tid; e < E; e += NT) {
-        const int lx = e % EW, ly = e / EW;
-        const int gi = wrap(i0 + lx - 1, Nx), gj = wrap(j0 + ly - 1, Ny);
-        const size_t off = (size_t)gj * P + gi;
-        const uint8_t m = a.mask[off];
-        const T eta = se[off];
-        s_m[e] = m;
-        s_u[e] = su[off];
-        s_v[e] = sv[off];
-        s_eta[e] = eta;
-        const T He = fld(a.He, off, gj, OE(m) != 0);
-        T D;
-        if (FN) D = OE(m) ? (eta + He) : T(1);   // computeD: D = eta + H on ocean eta, 1 elsewhere
-        else D = He;                              // linear variants: D aliases H_eta
-        s_D[e] = D;
+    // ---- phase 1: state with a one-cell halo (periodic indices) into LDS.
+    // Every global load of the block is issued before the first LDS store, so
+    // one memory latency covers the whole tile and its per-row tables.
+    constexpr int NE = (E + NT - 1) / NT;
+    int rm[NE];
+    T ru[NE], rv[NE], re[NE], rHo[NE], rHl[NE], rch[NTR][NE], rcb[NTR][NE];
 #pragma unroll
-        for (int n = 0; n < NTRC; ++n) {
-            const T* tb = st + (PL_TRC + PL_PER_TRC * n) * pl;
-            const T ch = tb[d.cur * pl + off];
-            s_CH[n][e] = ch;
-            // C = CH / h (TRC_tracer_advance), except the host-held C right after a hand-over
-            s_C[n][e] = ((d.cbuf >> n) & 1) ? tb[5 * pl + off] : ch / D;
+    for (int q = 0; q < NE; ++q) {
+        const int e = tid + q * NT;
+        if (e < E) {
+            const int lx = e % EW, ly = e / EW;
+            const int gi = wrap(i0 + lx - 1, Nx), gj = wrap(j0 + ly - 1, Ny);
+            const size_t off = (size_t)gj * P + gi;
+            rm[q] = a.mask[off];
+            ru[q] = su[off];
+            rv[q] = sv[off];
+            re[q] = se[off];
+            // H_eta at an ocean (rHo) and a land (rHl) point of this cell's row
+            rHo[q] = a.He.full ? a.He.full[off] : a.He.rows[2 * gj];
+            rHl[q] = a.He.full ? rHo[q] : a.He.rows[2 * gj + 1];
+#pragma unroll
+            for (int n = 0; n < NTRC; ++n) {
+                const T* tb = st + (PL_TRC + PL_PER_TRC * n) * pl;
+                rch[n][q] = tb[d.cur * pl + off];
+                rcb[n][q] = ((d.cbuf >> n) & 1) ? tb[5 * pl + off] : T(0);
+            }
         }
     }
+    // per-row tables of rows j0-1 .. j0+TY for phase 2 (cu also of row j0-2)
+    T tz0 = T(0), tz1 = T(0), tcu = T(0), tcH = T(0), tfH = T(0), tcu0 = T(0), tpc = T(0);
+    if (tid < EH) {
+        const int gj = wrap(j0 + tid - 1, Ny);
+        tz0 = a.za[2 * gj];
+        tz1 = a.za[2 * gj + 1];
+        tcu = a.cu[gj];
+        tcH = a.cH[gj];
+        tfH = a.fH[gj];
+    }
+    if (tid == NT - 1) tcu0 = a.cu[wrap(j0 - 2, Ny)];
+    const bool has_pc = a.mix && tid < EH * 24;
+    if (has_pc) {
+        const int r = tid / 24, c = tid % 24;
+        const int gj = wrap(j0 + r - 1, Ny);
+        tpc = c < 12 ? a.pltc[gj * 12 + c] : a.pllc[gj * 12 + c - 12];
+    }
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+        const int e = tid + q * NT;
+        if (e < E) {
+            const int m = rm[q];
+            const T eta = re[q];
+            s_m[e] = (uint8_t)m;
+            s_u[e] = ru[q];
+            s_v[e] = rv[q];
+            s_eta[e] = eta;
+            const T He = OE(m) ? rHo[q] : rHl[q];
+            T D;
+            if (FN) D = OE(m) ? (eta + He) : T(1);   // computeD: D = eta + H on ocean eta, 1 elsewhere
+            else D = He;                              // linear variants: D aliases H_eta
+            s_D[e] = D;
+#pragma unroll
+            for (int n = 0; n < NTRC; ++n) {
+                s_CH[n][e] = rch[n][q];
+                // C = CH / h (TRC_tracer_advance), except the host-held C right after a hand-over
+                s_C[n][e] = ((d.cbuf >> n) & 1) ? rcb[n][q] : dv(rch[n][q], D);
+            }
+        }
+    }
+    if (tid < EH) {
+        lds.za[tid][0] = tz0;
+        lds.za[tid][1] = tz1;
+        lds.cu[tid + 1] = tcu;
+        lds.cH[tid] = tcH;
+        lds.fH[tid] = tfH;
+    }
+    if (tid == NT - 1) lds.cu[0] = tcu0;
+    if (has_pc) {
+        const int r = tid / 24, c = tid % 24;
+        if (c < 12) lds.pltc[r][c] = tpc;
+        else lds.pllc[r][c - 12] = tpc;
+    }
+    if (EH * 24 > NT && a.mix)   // tiles taller than 8 rows: the remaining table rows
+        for (int q = tid + NT; q < EH * 24; q += NT) {
+            const int r = q / 24, c = q % 24;
+            const int gj = wrap(j0 + r - 1, Ny);
+            if (c < 12) lds.pltc[r][c] = a.pltc[gj * 12 + c];
+            else lds.pllc[r][c - 12] = a.pllc[gj * 12 + c - 12];
+        }
     __syncthreads();
 
     // ---- phase 2: derived fields on the extended tile
@@ -237,21 +364,21 @@ step_kernel(const Dyn<T> d) {
                 const int gjm = gj == 0 ? Ny - 1 : gj - 1;
                 T za_ = T(0), zb = T(0);
                 if (OV(s_m[kw]) + OV(m) != 0)
-                    za_ = a.za[2 * gj + (OH(m) ? 0 : 1)] * (s_v[e] - s_v[kw]);
+                    za_ = lds.za[ly][OH(m) ? 0 : 1] * (s_v[e] - s_v[kw]);
                 if (OU(m) + OU(s_m[ks]) != 0) {
                     const T bcH = OH(m) ? T(1) : a.lbc;
-                    zb = bcH * (a.cu[gj] * s_u[e] - a.cu[gjm] * s_u[ks]) / a.AdT;
+                    zb = dv(bcH * (lds.cu[ly + 1] * s_u[e] - lds.cu[ly] * s_u[ks]), a.AdT);
                 }
-                zeta = za_ - zb / a.cH[gj];
+                zeta = za_ - dv(zb, lds.cH[ly]);
             }
             // computePotVort (src/swm_timestep_module.f90:230-257)
             const T pD = Dh > a.eps ? Dh : a.eps;
-            const T Pot = FN ? (a.fH[gj] + zeta) / pD : a.fH[gj] / pD;
+            const T Pot = FN ? dv(lds.fH[ly] + zeta, pD) : dv(lds.fH[ly], pD);
             s_Pot[e] = Pot;
             // SWM_LateralMixing_compute_p, P_lambda-theta on ocean H points
             T plt = T(0);
             if (a.mix && OH(m)) {
-                const T* c = a.pltc + (gj * 4 + OU(m) * 2 + OU(s_m[ks])) * 3;
+                const T* c = &lds.pltc[ly][(OU(m) * 2 + OU(s_m[ks])) * 3];
                 plt = Dh * ((c[0] * (s_v[e] - s_v[kw]) + c[1] * s_u[e]) + c[2] * s_u[ks]);
             }
             s_plt[e] = plt;
@@ -284,7 +411,7 @@ step_kernel(const Dyn<T> d) {
             // P_lambda-lambda on ocean eta points
             T pll = T(0);
             if (a.mix && OE(m)) {
-                const T* c = a.pllc + (gj * 4 + OV(s_m[kn]) * 2 + OV(m)) * 3;
+                const T* c = &lds.pllc[ly][(OV(s_m[kn]) * 2 + OV(m)) * 3];
                 pll = s_D[e] * ((c[0] * (s_u[ke] - s_u[e]) + c[1] * s_v[kn]) + c[2] * s_v[e]);
             }
             s_pll[e] = pll;
@@ -294,12 +421,33 @@ step_kernel(const Dyn<T> d) {
     __syncthreads();
 
     // ---- phase 3: tendencies, AB integration, clamp, tracers
+    // With 64-wide tiles every wavefront owns one output row: the row index is
+    // uniform and the per-row tables are read with scalar loads.
+    constexpr bool UROW = (TX == 64 && NO == 1);
 #pragma unroll
     for (int q = 0; q < NO; ++q) {
         const int o = tid + q * NT;
         const int lx = o % TX + 1, ly = o / TX + 1;
-        const int i = i0 + lx - 1, j = j0 + ly - 1;
+        const int i = i0 + lx - 1;
+        const int j = UROW ? __builtin_amdgcn_readfirstlane(j0 + ly - 1) : j0 + ly - 1;
         if (o >= TX * TY || i >= Nx || j >= Ny) continue;
+        auto rt = [&](const T* p, int idx) -> T {   // per-row table value
+            if constexpr (UROW) return ldc(p, (size_t)idx);
+            else return p[idx];
+        };
+        auto fl = [&](const Fld<T>& f, size_t at, bool ocean) -> T {   // static field at (i, j)
+            if constexpr (UROW) {
+                if (f.full) return f.full[at];
+                const T fo = ldc(f.rows, 2 * (size_t)j), fl_ = ldc(f.rows, 2 * (size_t)j + 1);
+                return ocean ? fo : fl_;
+            } else {
+                return fld(f, at, j, ocean);
+            }
+        };
+        auto ta = [&](bool neg, int nummask, int den, const T* tab) -> T {
+            if constexpr (UROW) return trc_adv_r<T>(neg, nummask, den, tab, j);
+            else return trc_adv<T>(neg, nummask, den, tab, j);
+        };
         const size_t off = (size_t)j * P + i;
         const int k = ly * EW + lx;
         const int ke = k + 1, kw = k - 1, kn = k + EW, ks = k - EW, knw = kn - 1, kse = ks + 1;
@@ -310,44 +458,46 @@ step_kernel(const Dyn<T> d) {
         const T Dv = interp_noland2(s_D[k], s_D[ks], oe, OE(s_m[ks]));
 
         // SWM_forcing_update (src/swm_forcing_module.f90:135-170)
-        T Fe = fld(a.Fec, off, j, oe);
-        if (a.has_oe) Fe = Fe + (T(2) * (fld(a.oRe, off, j, oe) * d.oc[2] - fld(a.oIe, off, j, oe) * d.os[2])) * T(oe);
-        T Fx = fld(a.Fxc, off, j, ou);
+        T Fe = fl(a.Fec, off, oe);
+        if (a.has_oe) Fe = Fe + (T(2) * (fl(a.oRe, off, oe) * d.oc[2] - fl(a.oIe, off, oe) * d.os[2])) * T(oe);
+        T Fx = fl(a.Fxc, off, ou);
         if (a.has_taux && ou) {
-            const T tx = fld(a.taux, off, j, true);
-            Fx = Fx + (a.has_tau_scale ? a.tau_scale * tx : tx) / (a.RHO0 * Du);
+            const T tx = fl(a.taux, off, true);
+            Fx = Fx + dv(a.has_tau_scale ? a.tau_scale * tx : tx, a.RHO0 * Du);
         }
-        if (a.has_ox) Fx = Fx + (T(2) * (fld(a.oRx, off, j, ou) * d.oc[0] - fld(a.oIx, off, j, ou) * d.os[0])) * T(ou);
-        T Fy = fld(a.Fyc, off, j, ov);
+        if (a.has_ox) Fx = Fx + (T(2) * (fl(a.oRx, off, ou) * d.oc[0] - fl(a.oIx, off, ou) * d.os[0])) * T(ou);
+        T Fy = fl(a.Fyc, off, ov);
         if (a.has_tauy && ov) {
-            const T ty = fld(a.tauy, off, j, true);
-            Fy = Fy + (a.has_tau_scale ? a.tau_scale * ty : ty) / (a.RHO0 * Dv);
+            const T ty = fl(a.tauy, off, true);
+            Fy = Fy + dv(a.has_tau_scale ? a.tau_scale * ty : ty, a.RHO0 * Dv);
         }
-        if (a.has_oy) Fy = Fy + (T(2) * (fld(a.oRy, off, j, ov) * d.oc[1] - fld(a.oIy, off, j, ov) * d.os[1])) * T(ov);
+        if (a.has_oy) Fy = Fy + (T(2) * (fl(a.oRy, off, ov) * d.oc[1] - fl(a.oIy, off, ov) * d.os[1])) * T(ov);
 
         // SWM_timestep_nonlinear (src/swm_timestep_module.f90:349-443)
         T Ge = T(0), Gu = T(0), Gv = T(0), Lu = T(0), Lv = T(0);
         if (oe) {
-            const T t1 = -(s_MU[ke] - s_MU[k]) / a.dL;
-            const T t2 = (s_MV[kn] * a.cv[jp] - s_MV[k] * a.cv[j]) / a.dT;
-            Ge = (t1 - t2) / a.ace[j] + Fe;
+            const T t1 = dv(-(s_MU[ke] - s_MU[k]), a.dL);
+            const T t2 = dv(s_MV[kn] * rt(a.cv, jp) - s_MV[k] * rt(a.cv, j), a.dT);
+            Ge = dv(t1 - t2, rt(a.ace, j)) + Fe;
         }
         if (ou) {
             const T P2 = T(0.5) * s_Pot[kn] + T(0.5) * s_Pot[k];
             const T M4 = ((T(0.25) * s_MV[kn] + T(0.25) * s_MV[kw]) + T(0.25) * s_MV[knw]) + T(0.25) * s_MV[k];
             T g = P2 * M4;
-            g = g - ((s_Ed[k] - s_Ed[kw]) / a.edu[j]);
+            g = g - dv(s_Ed[k] - s_Ed[kw], rt(a.edu, j));
             if (a.quad) {
                 const T vi = ((T(0.25) * s_v[kn] + T(0.25) * s_v[kw]) + T(0.25) * s_v[knw]) + T(0.25) * s_v[k];
                 const T uc = s_u[k];
-                T qf = fld(a.gsu, off, j, true) * sqrt(uc * uc + (vi * vi)) * uc;
-                if (a.barotropic) qf = qf / fld(a.Hu, off, j, true);
+                T qf = fl(a.gsu, off, true) * sqrt(uc * uc + (vi * vi)) * uc;
+                if (a.barotropic) qf = dv(qf, fl(a.Hu, off, true));
                 g = g - qf;
             }
             if (a.mix) {
                 const int ot = (oh + OH(s_m[kn])) < 1 ? 1 : (oh + OH(s_m[kn]));
-                const T* c = a.lmu + (j * 2 + ot - 1) * 3;
-                Lu = ((c[0] * (s_pll[k] - s_pll[kw]) + c[1] * s_plt[kn]) + c[2] * s_plt[k]) / Du;
+                const T* c = a.lmu + j * 6;
+                const T c0 = ot == 1 ? rt(c, 0) : rt(c, 3), c1 = ot == 1 ? rt(c, 1) : rt(c, 4),
+                        c2 = ot == 1 ? rt(c, 2) : rt(c, 5);
+                Lu = dv((c0 * (s_pll[k] - s_pll[kw]) + c1 * s_plt[kn]) + c2 * s_plt[k], Du);
                 g = g + Lu;
             }
             Gu = g + Fx;
@@ -356,18 +506,20 @@ step_kernel(const Dyn<T> d) {
             const T P2 = T(0.5) * s_Pot[ke] + T(0.5) * s_Pot[k];
             const T M4 = ((T(0.25) * s_MU[ke] + T(0.25) * s_MU[ks]) + T(0.25) * s_MU[k]) + T(0.25) * s_MU[kse];
             T g = -(P2 * M4);
-            g = g - ((s_Ed[k] - s_Ed[ks]) / a.AdT);
+            g = g - dv(s_Ed[k] - s_Ed[ks], a.AdT);
             if (a.quad) {
                 const T ui = ((T(0.25) * s_u[ke] + T(0.25) * s_u[ks]) + T(0.25) * s_u[k]) + T(0.25) * s_u[kse];
                 const T vc = s_v[k];
-                T qf = fld(a.gsv, off, j, true) * sqrt(vc * vc + (ui * ui)) * vc;
-                if (a.barotropic) qf = qf / fld(a.Hv, off, j, true);
+                T qf = fl(a.gsv, off, true) * sqrt(vc * vc + (ui * ui)) * vc;
+                if (a.barotropic) qf = dv(qf, fl(a.Hv, off, true));
                 g = g - qf;
             }
             if (a.mix) {
                 const int ot = (oe + OE(s_m[ks])) < 1 ? 1 : (oe + OE(s_m[ks]));
-                const T* c = a.lmv + (j * 2 + ot - 1) * 3;
-                Lv = ((c[0] * (s_plt[ke] - s_plt[k]) + c[1] * s_pll[k]) + c[2] * s_pll[ks]) / Dv;
+                const T* c = a.lmv + j * 6;
+                const T c0 = ot == 1 ? rt(c, 0) : rt(c, 3), c1 = ot == 1 ? rt(c, 1) : rt(c, 4),
+                        c2 = ot == 1 ? rt(c, 2) : rt(c, 5);
+                Lv = dv((c0 * (s_plt[ke] - s_plt[k]) + c1 * s_pll[k]) + c2 * s_pll[ks], Dv);
                 g = g + Lv;
             }
             Gv = g + Fy;
@@ -392,24 +544,24 @@ step_kernel(const Dyn<T> d) {
         const T Gv3 = ov ? Gv : pg[q][3];
         const T Ge3 = oe ? Ge : pg[q][5];
         T un = s_u[k], vn = s_v[k], en = s_eta[k];
-        un = un + d.dtc1 * pg[q][0];
-        vn = vn + d.dtc1 * pg[q][2];
-        en = en + d.dtc1 * pg[q][4];
+        un = madd(un, d.dtc1, pg[q][0]);
+        vn = madd(vn, d.dtc1, pg[q][2]);
+        en = madd(en, d.dtc1, pg[q][4]);
         if (d.tstep >= 2) {
-            un = un + d.dtc2 * pg[q][1];
-            vn = vn + d.dtc2 * pg[q][3];
-            en = en + d.dtc2 * pg[q][5];
+            un = madd(un, d.dtc2, pg[q][1]);
+            vn = madd(vn, d.dtc2, pg[q][3]);
+            en = madd(en, d.dtc2, pg[q][5]);
         }
         if (d.tstep >= 3) {
-            un = un + d.dtc3 * Gu3;
-            vn = vn + d.dtc3 * Gv3;
-            en = en + d.dtc3 * Ge3;
+            un = madd(un, d.dtc3, Gu3);
+            vn = madd(vn, d.dtc3, Gv3);
+            en = madd(en, d.dtc3, Ge3);
         }
-        un = un / fld(a.imu, off, j, ou);
-        vn = vn / fld(a.imv, off, j, ov);
-        en = en / fld(a.ime, off, j, oe);
+        un = dv(un, fl(a.imu, off, ou));
+        vn = dv(vn, fl(a.imv, off, ov));
+        en = dv(en, fl(a.ime, off, oe));
         if (FN) {   // SWM_timestep_step: where (eta < minD - H) eta = minD - H
-            const T lim = a.minD - fld(a.He, off, j, oe);
+            const T lim = a.minD - fl(a.He, off, oe);
             if (en < lim) en = lim;
         }
         st[(PL_U + 1 - d.cur) * pl + off] = un;
@@ -435,22 +587,22 @@ step_kernel(const Dyn<T> d) {
                 const T kap = a.kappa[n];
                 const T hue = interp_noland2(s_D[ke], s_D[k], oee, oe);
                 const T hvn = interp_noland2(s_D[kn], s_D[k], oen, oe);
-                const T c9 = oue ? a.t9[j] : T(0.0);
-                const T c10 = ou ? -a.t9[j] : T(-0.0);
-                const T c11 = ovn ? a.t11[j] : T(0.0);
-                const T c12 = ov ? -a.t12[j] : T(-0.0);
+                const T c9 = oue ? rt(a.t9, j) : T(0.0);
+                const T c10 = ou ? -rt(a.t9, j) : T(-0.0);
+                const T c11 = ovn ? rt(a.t11, j) : T(0.0);
+                const T c12 = ov ? -rt(a.t12, j) : T(-0.0);
                 diff = ((((kap * hue) * (Ce - C)) * c9 + ((kap * Du) * (C - Cw)) * c10) +
                         ((kap * hvn) * (Cn - C)) * c11) + ((kap * Dv) * (C - Cs)) * c12;
             }
             const T forcing = C * Fe;
-            const T c1 = trc_adv<T>(true, oue & oee, oee + oe, a.tA, j);
-            const T c2 = trc_adv<T>(false, ou & oew, oew + oe, a.tA, j);
-            const T c3 = trc_adv<T>(true, ovn & oen, oen + oe, a.tB, j);
-            const T c4 = trc_adv<T>(false, ov & oes, oes + oe, a.tC, j);
-            const T c5 = trc_adv<T>(true, oue & oe, oee + oe, a.tA, j);
-            const T c6 = trc_adv<T>(false, ou & oe, oew + oe, a.tA, j);
-            const T c7 = trc_adv<T>(true, ovn & oe, oen + oe, a.tB, j);
-            const T c8 = trc_adv<T>(false, ov & oe, oes + oe, a.tC, j);
+            const T c1 = ta(true, oue & oee, oee + oe, a.tA);
+            const T c2 = ta(false, ou & oew, oew + oe, a.tA);
+            const T c3 = ta(true, ovn & oen, oen + oe, a.tB);
+            const T c4 = ta(false, ov & oes, oes + oe, a.tC);
+            const T c5 = ta(true, oue & oe, oee + oe, a.tA);
+            const T c6 = ta(false, ou & oe, oew + oe, a.tA);
+            const T c7 = ta(true, ovn & oe, oen + oe, a.tB);
+            const T c8 = ta(false, ov & oe, oes + oe, a.tC);
             T g = (CHe * ume) * c1 + (CHw * um) * c2;
             g = g + (CHn * vmn) * c3;
             g = g + (CHs * vm) * c4;
@@ -459,16 +611,102 @@ step_kernel(const Dyn<T> d) {
             g = g + (CH * vmn) * c7;
             g = g + (CH * vm) * c8;
             g = g + diff;
-            g = g - (fld(a.gC[n], off, j, oe) * s_D[k]) * (C - fld(a.C0[n], off, j, oe));
+            g = g - (fl(a.gC[n], off, oe) * s_D[k]) * (C - fl(a.C0[n], off, oe));
             g = g + forcing;
             T chn = CH + d.dtc1 * pt[q][n][0];
-            if (d.tstep >= 2) chn = chn + d.dtc2 * pt[q][n][1];
-            if (d.tstep >= 3) chn = chn + d.dtc3 * g;
-            chn = chn / fld(a.timpl[n], off, j, oe);
+            if (d.tstep >= 2) chn = madd(chn, d.dtc2, pt[q][n][1]);
+            if (d.tstep >= 3) chn = madd(chn, d.dtc3, g);
+            chn = dv(chn, fl(a.timpl[n], off, oe));
             tb[(1 - d.cur) * pl + off] = chn;
             tb[(2 + r2) * pl + off] = g;
         }
     }
+}
+
+template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG>
+__global__ void __launch_bounds__(256)
+step_kernel(const Dyn<T> d) {
+    __shared__ TileLds<T, (NTRC > 0 ? NTRC : 1), (TX + 2) * (TY + 2), TY + 2> lds;
+    tile_body<T, VAR, NTRC, TX, TY, DIAG>(d, xcd_tile(blockIdx.x, d.nblk), lds);
+}
+
+// ---------------------------------------------------------------------------
+// Batched step: many independent domains of uneven size advance by one step in
+// one launch.  Work item w is tile (w - wbase) of member owner[w]; the
+// per-member step state (level, AB ring slot, itt, steps left) lives in
+// device memory and batch_advance moves it on after each step, so a captured
+// graph of the two launches replays any number of steps.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Member {
+    const Statics<T>* s;
+    T* state;
+    long long plane;
+    int ntx;          // tile columns
+    int ntile;        // tiles of this member
+    int wbase;        // first work item of this member
+    int ntr;          // tracers
+    T dtc[3][3];      // dt*ab_coeff(ti, tstep) for tstep = 1..3
+};
+
+struct MemberStep {
+    long long itt;        // steps done
+    long long remaining;  // steps still to do
+    int cur, ring, cbuf, pad;
+};
+
+template <typename T, int VAR, int NTRC_MAX, int TX, int TY>
+__global__ void __launch_bounds__(256)
+batch_kernel(const Member<T>* __restrict__ mem, const MemberStep* __restrict__ stp, const int* __restrict__ owner,
+             int nwork) {
+    __shared__ TileLds<T, (NTRC_MAX > 0 ? NTRC_MAX : 1), (TX + 2) * (TY + 2), TY + 2> lds;
+    const int w = xcd_tile(blockIdx.x, nwork);
+    const int lo = owner[w];   // member whose work range holds w
+    const Member<T>& m = mem[lo];
+    const MemberStep ms = stp[lo];
+    if (ms.remaining <= 0) return;   // whole block: the member is done
+    Dyn<T> d{};
+    d.s = m.s;
+    d.state = m.state;
+    d.plane = m.plane;
+    d.ntx = m.ntx;
+    d.nblk = m.ntile;
+    d.ty = TY;
+    d.cur = ms.cur;
+    d.ring = ms.ring;
+    d.cbuf = ms.cbuf;
+    const long long itt = ms.itt + 1;
+    const int tstep = itt < 3 ? (int)itt : 3;
+    d.tstep = tstep;
+    d.dtc1 = m.dtc[tstep - 1][0];
+    d.dtc2 = m.dtc[tstep - 1][1];
+    d.dtc3 = m.dtc[tstep - 1][2];
+    const int t = w - m.wbase;
+    if constexpr (NTRC_MAX == 0) {
+        tile_body<T, VAR, 0, TX, TY, false>(d, t, lds);
+    } else if constexpr (NTRC_MAX == 1) {
+        if (m.ntr == 0) tile_body<T, VAR, 0, TX, TY, false>(d, t, lds);
+        else tile_body<T, VAR, 1, TX, TY, false>(d, t, lds);
+    } else {
+        if (m.ntr == 0) tile_body<T, VAR, 0, TX, TY, false>(d, t, lds);
+        else if (m.ntr == 1) tile_body<T, VAR, 1, TX, TY, false>(d, t, lds);
+        else tile_body<T, VAR, 2, TX, TY, false>(d, t, lds);
+    }
+}
+
+// model_advance of every member that stepped: N0 <- N0p1, G ring moves on.
+__global__ void __launch_bounds__(256)
+batch_advance(MemberStep* stp, int nmem) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nmem) return;
+    MemberStep s = stp[k];
+    if (s.remaining <= 0) return;
+    s.itt += 1;
+    s.remaining -= 1;
+    s.cur = 1 - s.cur;
+    s.ring = s.ring == 2 ? 0 : s.ring + 1;
+    s.cbuf = 0;
+    stp[k] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -501,38 +739,6 @@ __device__ __forceinline__ double lane_right(double x) {
 }
 __device__ __forceinline__ float lane_left(float x) { return __int_as_float(lane_left_i(__float_as_int(x))); }
 __device__ __forceinline__ float lane_right(float x) { return __int_as_float(lane_right_i(__float_as_int(x))); }
-
-// Loads for the marching kernel: per-lane values through a global pointer with
-// a 32-bit lane index (base address stays in SGPRs), and uniform values of the
-// read-only per-row tables through the constant address space (scalar loads).
-template <typename T>
-__device__ __forceinline__ T ldg(const T* p, unsigned i) {
-    return ((const __attribute__((address_space(1))) T*)p)[i];
-}
-template <typename T>
-__device__ __forceinline__ void stg(T* p, unsigned i, T v) {
-    ((__attribute__((address_space(1))) T*)p)[i] = v;
-}
-template <typename T>
-__device__ __forceinline__ T ldc(const T* p, size_t i) {
-    return ((const __attribute__((address_space(4))) T*)p)[i];
-}
-// fld() for a uniform row: ro = row offset, j = row index
-template <typename T>
-__device__ __forceinline__ T fld_r(const Fld<T>& f, size_t ro, unsigned gi, int j, bool ocean) {
-    if (f.full) return ldg(f.full + ro, gi);
-    const T o = ldc(f.rows, 2 * (size_t)j), l = ldc(f.rows, 2 * (size_t)j + 1);
-    return ocean ? o : l;
-}
-// trc_adv() for a uniform row
-template <typename T>
-__device__ __forceinline__ T trc_adv_r(bool neg, int nummask, int den, const T* tab, int j) {
-    const T v1 = ldc(tab, 2 * (size_t)j), v2 = ldc(tab, 2 * (size_t)j + 1);
-    if (den == 0) return __builtin_nan("");
-    if (!nummask) return neg ? T(-0.0) : T(0.0);
-    const T v = den == 1 ? v1 : v2;
-    return neg ? -v : v;
-}
 
 template <typename T, int NTR>
 struct MRaw {            // one loaded row at this lane
@@ -622,7 +828,7 @@ march_kernel(const Dyn<T> d) {
         K.D = FN ? (OE(K.m) ? (K.eta + He) : T(1)) : He;   // computeD; linear variants alias H_eta
 #pragma unroll
         for (int n = 0; n < NTRC; ++n)   // C = CH / h, or the host-held C right after a hand-over
-            K.C[n] = ((d.cbuf >> n) & 1) ? Cb[n] : K.CH[n] / K.D;
+            K.C[n] = ((d.cbuf >> n) & 1) ? Cb[n] : dv(K.CH[n], K.D);
         const int KmL = lane_left_i(K.m);
         const T KDL = lane_left(K.D);
         K.Du = interp_noland2(K.D, KDL, OE(K.m), OE(KmL));     // eta2u_noland
@@ -650,12 +856,12 @@ march_kernel(const Dyn<T> d) {
                 }
                 if (OU(K.m) + OU(P1.m) != 0) {
                     const T bcH = OH(K.m) ? T(1) : a.lbc;
-                    zb = bcH * (ldc(a.cu, gk) * K.u - ldc(a.cu, gkm) * P1.u) / a.AdT;
+                    zb = dv(bcH * (ldc(a.cu, gk) * K.u - ldc(a.cu, gkm) * P1.u), a.AdT);
                 }
-                zeta = za_ - zb / ldc(a.cH, gk);
+                zeta = za_ - dv(zb, ldc(a.cH, gk));
             }
             const T pD = Dh > a.eps ? Dh : a.eps;                      // computePotVort
-            K.Pot = FN ? (ldc(a.fH, gk) + zeta) / pD : ldc(a.fH, gk) / pD;
+            K.Pot = FN ? dv(ldc(a.fH, gk) + zeta, pD) : dv(ldc(a.fH, gk), pD);
             T plt = T(0);                                              // P_lambda-theta on ocean H
             if (a.mix && OH(K.m)) {
                 const T* c = a.pltc + (size_t)gk * 12;
@@ -713,32 +919,32 @@ march_kernel(const Dyn<T> d) {
                     T Fx = fld_r(a.Fxc, ro, gi, j, ou);
                     if (a.has_taux && ou) {
                         const T tx = fld_r(a.taux, ro, gi, j, true);
-                        Fx = Fx + (a.has_tau_scale ? a.tau_scale * tx : tx) / (a.RHO0 * Du);
+                        Fx = Fx + dv(a.has_tau_scale ? a.tau_scale * tx : tx, a.RHO0 * Du);
                     }
                     if (a.has_ox) Fx = Fx + (T(2) * (fld_r(a.oRx, ro, gi, j, ou) * d.oc[0] - fld_r(a.oIx, ro, gi, j, ou) * d.os[0])) * T(ou);
                     T Fy = fld_r(a.Fyc, ro, gi, j, ov);
                     if (a.has_tauy && ov) {
                         const T ty = fld_r(a.tauy, ro, gi, j, true);
-                        Fy = Fy + (a.has_tau_scale ? a.tau_scale * ty : ty) / (a.RHO0 * Dv);
+                        Fy = Fy + dv(a.has_tau_scale ? a.tau_scale * ty : ty, a.RHO0 * Dv);
                     }
                     if (a.has_oy) Fy = Fy + (T(2) * (fld_r(a.oRy, ro, gi, j, ov) * d.oc[1] - fld_r(a.oIy, ro, gi, j, ov) * d.os[1])) * T(ov);
 
                     // SWM_timestep_nonlinear
                     T Ge = T(0), Gu = T(0), Gv = T(0);
                     if (oe) {
-                        const T t1 = -(P1MUR - P1.MU) / a.dL;
-                        const T t2 = (K.MV * ldc(a.cv, jp) - P1.MV * ldc(a.cv, j)) / a.dT;
-                        Ge = (t1 - t2) / ldc(a.ace, j) + Fe;
+                        const T t1 = dv(-(P1MUR - P1.MU), a.dL);
+                        const T t2 = dv(K.MV * ldc(a.cv, jp) - P1.MV * ldc(a.cv, j), a.dT);
+                        Ge = dv(t1 - t2, ldc(a.ace, j)) + Fe;
                     }
                     if (ou) {
                         const T P2u = T(0.5) * K.Pot + T(0.5) * P1.Pot;
                         const T M4 = ((T(0.25) * K.MV + T(0.25) * P1MVL) + T(0.25) * KMVL) + T(0.25) * P1.MV;
                         T gg = P2u * M4;
-                        gg = gg - ((P1.Ed - P1EdL) / ldc(a.edu, j));
+                        gg = gg - dv(P1.Ed - P1EdL, ldc(a.edu, j));
                         if (a.quad) {
                             const T vi = ((T(0.25) * K.v + T(0.25) * P1vL) + T(0.25) * KvL) + T(0.25) * P1.v;
                             T qf = fld_r(a.gsu, ro, gi, j, true) * sqrt(P1.u * P1.u + (vi * vi)) * P1.u;
-                            if (a.barotropic) qf = qf / fld_r(a.Hu, ro, gi, j, true);
+                            if (a.barotropic) qf = dv(qf, fld_r(a.Hu, ro, gi, j, true));
                             gg = gg - qf;
                         }
                         if (a.mix) {
@@ -746,7 +952,7 @@ march_kernel(const Dyn<T> d) {
                             const T* c = a.lmu + (size_t)j * 6;
                             const T c0 = ot == 1 ? ldc(c, 0) : ldc(c, 3), c1 = ot == 1 ? ldc(c, 1) : ldc(c, 4),
                                     c2 = ot == 1 ? ldc(c, 2) : ldc(c, 5);
-                            gg = gg + ((c0 * (P1.pll - P1pllL) + c1 * K.plt) + c2 * P1.plt) / Du;
+                            gg = gg + dv((c0 * (P1.pll - P1pllL) + c1 * K.plt) + c2 * P1.plt, Du);
                         }
                         Gu = gg + Fx;
                     }
@@ -754,11 +960,11 @@ march_kernel(const Dyn<T> d) {
                         const T P2v = T(0.5) * P1PotR + T(0.5) * P1.Pot;
                         const T M4 = ((T(0.25) * P1MUR + T(0.25) * P2.MU) + T(0.25) * P1.MU) + T(0.25) * P2MUR;
                         T gg = -(P2v * M4);
-                        gg = gg - ((P1.Ed - P2.Ed) / a.AdT);
+                        gg = gg - dv(P1.Ed - P2.Ed, a.AdT);
                         if (a.quad) {
                             const T ui = ((T(0.25) * P1uR + T(0.25) * P2.u) + T(0.25) * P1.u) + T(0.25) * P2uR;
                             T qf = fld_r(a.gsv, ro, gi, j, true) * sqrt(P1.v * P1.v + (ui * ui)) * P1.v;
-                            if (a.barotropic) qf = qf / fld_r(a.Hv, ro, gi, j, true);
+                            if (a.barotropic) qf = dv(qf, fld_r(a.Hv, ro, gi, j, true));
                             gg = gg - qf;
                         }
                         if (a.mix) {
@@ -766,7 +972,7 @@ march_kernel(const Dyn<T> d) {
                             const T* c = a.lmv + (size_t)j * 6;
                             const T c0 = ot == 1 ? ldc(c, 0) : ldc(c, 3), c1 = ot == 1 ? ldc(c, 1) : ldc(c, 4),
                                     c2 = ot == 1 ? ldc(c, 2) : ldc(c, 5);
-                            gg = gg + ((c0 * (P1pltR - P1.plt) + c1 * P1.pll) + c2 * P2.pll) / Dv;
+                            gg = gg + dv((c0 * (P1pltR - P1.plt) + c1 * P1.pll) + c2 * P2.pll, Dv);
                         }
                         Gv = gg + Fy;
                     }
@@ -775,22 +981,22 @@ march_kernel(const Dyn<T> d) {
                     const T Gv3 = ov ? Gv : g[3];
                     const T Ge3 = oe ? Ge : g[5];
                     T un = P1.u, vn = P1.v, en = P1.eta;
-                    un = un + d.dtc1 * g[0];
-                    vn = vn + d.dtc1 * g[2];
-                    en = en + d.dtc1 * g[4];
+                    un = madd(un, d.dtc1, g[0]);
+                    vn = madd(vn, d.dtc1, g[2]);
+                    en = madd(en, d.dtc1, g[4]);
                     if (d.tstep >= 2) {
-                        un = un + d.dtc2 * g[1];
-                        vn = vn + d.dtc2 * g[3];
-                        en = en + d.dtc2 * g[5];
+                        un = madd(un, d.dtc2, g[1]);
+                        vn = madd(vn, d.dtc2, g[3]);
+                        en = madd(en, d.dtc2, g[5]);
                     }
                     if (d.tstep >= 3) {
-                        un = un + d.dtc3 * Gu3;
-                        vn = vn + d.dtc3 * Gv3;
-                        en = en + d.dtc3 * Ge3;
+                        un = madd(un, d.dtc3, Gu3);
+                        vn = madd(vn, d.dtc3, Gv3);
+                        en = madd(en, d.dtc3, Ge3);
                     }
-                    un = un / fld_r(a.imu, ro, gi, j, ou);
-                    vn = vn / fld_r(a.imv, ro, gi, j, ov);
-                    en = en / fld_r(a.ime, ro, gi, j, oe);
+                    un = dv(un, fld_r(a.imu, ro, gi, j, ou));
+                    vn = dv(vn, fld_r(a.imv, ro, gi, j, ov));
+                    en = dv(en, fld_r(a.ime, ro, gi, j, oe));
                     if (FN) {   // where (eta < minD - H) eta = minD - H
                         const T lim = a.minD - fld_r(a.He, ro, gi, j, oe);
                         if (en < lim) en = lim;
@@ -845,9 +1051,9 @@ march_kernel(const Dyn<T> d) {
                         gg = gg - (fld_r(a.gC[n], ro, gi, j, oe) * P1.D) * (C - fld_r(a.C0[n], ro, gi, j, oe));
                         gg = gg + forcing;
                         T chn = CH + d.dtc1 * gt[n][0];
-                        if (d.tstep >= 2) chn = chn + d.dtc2 * gt[n][1];
-                        if (d.tstep >= 3) chn = chn + d.dtc3 * gg;
-                        chn = chn / fld_r(a.timpl[n], ro, gi, j, oe);
+                        if (d.tstep >= 2) chn = madd(chn, d.dtc2, gt[n][1]);
+                        if (d.tstep >= 3) chn = madd(chn, d.dtc3, gg);
+                        chn = dv(chn, fld_r(a.timpl[n], ro, gi, j, oe));
                         stg(tb + (1 - d.cur) * pl, gi, chn);
                         stg(tb + (2 + r2) * pl, gi, gg);
                     }

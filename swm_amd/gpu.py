@@ -31,6 +31,9 @@ EXPORTED = [
     "swm_gpu_create", "swm_gpu_upload_state", "swm_gpu_add_tracer", "swm_gpu_step", "swm_gpu_itt",
     "swm_gpu_sync", "swm_gpu_sor", "swm_gpu_sor_fixed", "swm_gpu_divergence", "swm_gpu_sums",
     "swm_gpu_last_timing", "swm_gpu_last_error", "swm_gpu_destroy",
+    "swm_gpu_batch_create", "swm_gpu_batch_attach", "swm_gpu_batch_detach", "swm_gpu_batch_set_steps",
+    "swm_gpu_batch_remaining", "swm_gpu_batch_step", "swm_gpu_batch_last_timing", "swm_gpu_batch_last_error",
+    "swm_gpu_batch_destroy",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -101,6 +104,24 @@ def load_library(path: str = LIB_PATH):
     L.swm_gpu_last_timing.restype = C.c_int
     L.swm_gpu_last_error.argtypes = [p]
     L.swm_gpu_last_error.restype = C.c_char_p
+    L.swm_gpu_batch_create.argtypes = [C.POINTER(p)]
+    L.swm_gpu_batch_create.restype = C.c_int
+    L.swm_gpu_batch_attach.argtypes = [p, p, C.c_int64]
+    L.swm_gpu_batch_attach.restype = C.c_int
+    L.swm_gpu_batch_detach.argtypes = [p, p]
+    L.swm_gpu_batch_detach.restype = C.c_int
+    L.swm_gpu_batch_set_steps.argtypes = [p, p, C.c_int64]
+    L.swm_gpu_batch_set_steps.restype = C.c_int
+    L.swm_gpu_batch_remaining.argtypes = [p, p, C.POINTER(C.c_int64)]
+    L.swm_gpu_batch_remaining.restype = C.c_int
+    L.swm_gpu_batch_step.argtypes = [p, C.c_int64, C.POINTER(C.c_int32)]
+    L.swm_gpu_batch_step.restype = C.c_int
+    L.swm_gpu_batch_last_timing.argtypes = [p, _dp]
+    L.swm_gpu_batch_last_timing.restype = C.c_int
+    L.swm_gpu_batch_last_error.argtypes = [p]
+    L.swm_gpu_batch_last_error.restype = C.c_char_p
+    L.swm_gpu_batch_destroy.argtypes = [p]
+    L.swm_gpu_batch_destroy.restype = None
     L.swm_gpu_destroy.argtypes = [p]
     L.swm_gpu_destroy.restype = None
     _lib = L
@@ -296,3 +317,67 @@ class SwmGpu:
         if rc not in (0, 1):
             self._check(rc, "swm_gpu_sor_fixed")
         return chi, it.value, rr.value, rc == 0
+
+
+class SwmBatch:
+    """Batched ensemble (swm_gpu_batch_*): attached SwmGpu members advance
+    together, one launch per step covering every member's tiles."""
+
+    def __init__(self):
+        self.L = load_library()
+        h = C.c_void_p()
+        rc = self.L.swm_gpu_batch_create(C.byref(h))
+        if rc != 0:
+            raise SwmGpuError(f"swm_gpu_batch_create failed ({rc}): {self.L.swm_gpu_batch_last_error(None).decode()}")
+        self._h = h
+        self.members = []      # keeps attached contexts alive
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            raise SwmGpuError(f"{what} failed ({rc}): {self.L.swm_gpu_batch_last_error(self._h).decode()}")
+
+    def attach(self, m: "SwmGpu", nsteps: int):
+        self._check(self.L.swm_gpu_batch_attach(self._h, m._h, int(nsteps)), "swm_gpu_batch_attach")
+        self.members.append(m)
+
+    def detach(self, m: "SwmGpu"):
+        self._check(self.L.swm_gpu_batch_detach(self._h, m._h), "swm_gpu_batch_detach")
+        self.members.remove(m)
+
+    def set_steps(self, m: "SwmGpu", nsteps: int):
+        self._check(self.L.swm_gpu_batch_set_steps(self._h, m._h, int(nsteps)), "swm_gpu_batch_set_steps")
+
+    def remaining(self, m: "SwmGpu") -> int:
+        r = C.c_int64()
+        self._check(self.L.swm_gpu_batch_remaining(self._h, m._h, C.byref(r)), "swm_gpu_batch_remaining")
+        return r.value
+
+    def step(self, nsteps: int) -> int:
+        """Advance every member with steps left by up to nsteps; returns the
+        number of members with steps left afterwards."""
+        a = C.c_int32()
+        self._check(self.L.swm_gpu_batch_step(self._h, int(nsteps), C.byref(a)), "swm_gpu_batch_step")
+        return a.value
+
+    def last_ms(self) -> float:
+        t = C.c_double()
+        self._check(self.L.swm_gpu_batch_last_timing(self._h, C.byref(t)), "swm_gpu_batch_last_timing")
+        return t.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.swm_gpu_batch_destroy(self._h)
+            self._h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

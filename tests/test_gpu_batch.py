@@ -1,0 +1,98 @@
+"""Batched ensembles (swm_gpu_batch_*): every member of a batch gets the same
+result as the same context stepped alone, bit for bit, in FP64 and FP32;
+members stop at their own step counts; detach / re-attach keeps the state."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import same
+from swm_amd import configs, gpu, host
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["SWM_U", "SWM_V", "SWM_ETA", "G_U", "G_V", "G_ETA", "G_U_M1", "G_V_M1", "G_ETA_M1"]
+
+
+def members():
+    """Uneven domains: different sizes, with and without a tracer, islands or not."""
+    return [
+        configs.c2_nonlinear(seed=11, N=96, dt=10.0),
+        configs.c2_nonlinear(seed=12, N=130, dt=8.0, tracer=False),
+        host.finalize(configs.basin(70, 45, 5e3, 5e3, variant=host.FULLY_NONLINEAR, H0=800.0, dt=6.0, Ah=50.0,
+                                    f0=1e-4, beta=2e-11, seed=13, land_fraction=0.0)),
+        configs.c2_nonlinear(seed=14, N=64, dt=10.0),
+    ]
+
+
+def check_same(a, b, ntrc, tag):
+    for f in FIELDS:
+        assert same(a.get(f), b.get(f)), f"{tag} {f}"
+    for t in range(ntrc):
+        for f in ("CH", "G_CH", "G_CH_M1", "C"):
+            assert same(a.get(f, t), b.get(f, t)), f"{tag} tracer {f}"
+
+
+@pytest.mark.parametrize("precision", [gpu.FP64, gpu.FP32])
+def test_batch_members_equal_alone(gpu_lib, precision):
+    setups = members()
+    steps = [37, 25, 40, 9]
+    alone = [gpu.SwmGpu(s, precision).init_from_setup() for s in setups]
+    mixed = [gpu.SwmGpu(s, precision).init_from_setup() for s in setups]
+    for m, n in zip(alone, steps):
+        m.step(n)
+    with gpu.SwmBatch() as b:
+        for m, n in zip(mixed, steps):
+            b.attach(m, n)
+        active = b.step(15)
+        assert active == 3                       # the 9-step member is done
+        assert [b.remaining(m) for m in mixed] == [22, 10, 25, 0]
+        assert mixed[3].itt == 9                 # stopped at its own count
+        active = b.step(100)
+        assert active == 0
+        for m, a, s, n in zip(mixed, alone, setups, steps):
+            assert m.itt == a.itt == n
+            check_same(m, a, len(s.tracers), f"member {s.Nx}x{s.Ny}")
+    for m in alone + mixed:
+        m.close()
+
+
+def test_batch_matches_oracle_and_resumes_alone(gpu_lib):
+    """A member's batched run is bit-identical to the oracle; after detach it
+    continues alone, and re-attaching with a new count continues the batch."""
+    s = configs.c2_nonlinear(seed=21, N=80, dt=10.0)
+    o = oracle.Oracle(s)
+    m = gpu.SwmGpu(s).init_from_setup()
+    other = gpu.SwmGpu(configs.c2_nonlinear(seed=22, N=120, dt=10.0)).init_from_setup()
+    with gpu.SwmBatch() as b:
+        b.attach(m, 12)
+        b.attach(other, 30)
+        b.step(12)
+        o.step(12)
+        assert same(m.get("SWM_ETA"), o.get("eta")) and same(m.get("CH", 0), o.get("CH", 0))
+        with pytest.raises(gpu.SwmGpuError):
+            m.step(1)                            # attached contexts advance through the batch
+        b.detach(m)
+        m.step(5)
+        o.step(5)
+        assert same(m.get("SWM_U"), o.get("u"))
+        b.attach(m, 7)
+        b.set_steps(other, 7)
+        assert b.step(50) == 0
+        o.step(7)
+        assert m.itt == 24 and other.itt == 19
+        assert same(m.get("SWM_ETA"), o.get("eta")) and same(m.get("G_ETA"), o.get("G_eta"))
+    m.close()
+    other.close()
+
+
+def test_batch_rejects_mixed_precision(gpu_lib):
+    a = gpu.SwmGpu(configs.c2_nonlinear(seed=1, N=48, dt=10.0), gpu.FP64).init_from_setup()
+    f = gpu.SwmGpu(configs.c2_nonlinear(seed=1, N=48, dt=10.0), gpu.FP32).init_from_setup()
+    with gpu.SwmBatch() as b:
+        b.attach(a, 3)
+        with pytest.raises(gpu.SwmGpuError):
+            b.attach(f, 3)
+        b.step(3)
+        assert a.itt == 3
+    a.close()
+    f.close()
