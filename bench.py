@@ -168,6 +168,56 @@ def bench_ensemble(args):
     return 0
 
 
+def bench_serve(args):
+    """Serving stream: Poisson arrivals, continuous batching, FP32 fields with
+    FP64 sums; p50/p95/p99 completion latency, aggregate throughput, and the
+    eta error of a 1% sample replayed alone in FP64."""
+    import numpy as np
+    from swm_amd import gpu, serve
+    reqs = serve.request_stream(seed=0, rate=args.rate, duration=args.serve_seconds, keep_every=100)
+    offered = sum(r.cells * r.nsteps for r in reqs) / args.serve_seconds
+    srv = serve.Server(precision=gpu.FP32, hbm_budget=200e9, chunk=args.chunk)
+    st = srv.run(reqs, realtime=True)
+    errs = []
+    for r in reqs:
+        if not r.keep:
+            continue
+        s = serve.build_setup(r)
+        m = gpu.SwmGpu(s, gpu.FP64).init_from_setup()
+        m.step(r.nsteps)
+        oc = s.grids["eta"].ocean == 1
+        e64 = m.get("SWM_ETA")[oc]
+        m.close()
+        errs.append(float(np.abs(r.result["SWM_ETA"][oc] - e64).max() / max(np.abs(e64).max(), 1e-300)))
+    plain = plain_fp64_rate()
+    # plain path modelled as a FIFO queue served one request at a time at the
+    # plain FP64 rate measured above
+    t_free, lat_plain = 0.0, []
+    for r in sorted(reqs, key=lambda q: q.arrival):
+        start = max(t_free, r.arrival)
+        t_free = start + r.cells * r.nsteps / plain
+        lat_plain.append(t_free - r.arrival)
+    lat_plain = np.array(lat_plain)
+    out = {
+        "metric": METRIC, "value": st["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"serving stream: Poisson {args.rate:g} req/s for {args.serve_seconds:g} s, grids "
+                               f"log-uniform 64..2048 x 64..1024, 500..20000 steps, tracer p=0.5, FP32 fields, "
+                               f"200 GB admission budget, {args.chunk}-step chunks",
+                   "requests": st["requests"], "offered_cell_updates_per_s": offered},
+        "latency_s": {"p50": st["latency_p50_s"], "p95": st["latency_p95_s"], "p99": st["latency_p99_s"],
+                      "max": st["latency_max_s"]},
+        "plain_fifo_latency_s_modelled": {"p50": float(np.percentile(lat_plain, 50)),
+                                          "p99": float(np.percentile(lat_plain, 99))},
+        "wall_s": st["wall_s"], "gpu_busy_s": st["gpu_busy_s"], "chunks": st["chunks"],
+        "plain_fp64_c2_value": plain, "served_vs_plain_fp64": st["cell_updates_per_s"] / plain,
+        "fp64_replay": {"sample": len(errs), "eta_rel_maxnorm_max": max(errs) if errs else None},
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,9 +228,13 @@ def main(argv=None):
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["c2", "ensemble"], default="c2",
-                    help="c2: the 1024 x 1024 basin (default); ensemble: the batched 64-member ensemble")
+    ap.add_argument("--workload", choices=["c2", "ensemble", "serve"], default="c2",
+                    help="c2: the 1024 x 1024 basin (default); ensemble: the batched 64-member ensemble; "
+                         "serve: the request stream")
     ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
+    ap.add_argument("--serve-seconds", type=float, default=30.0, help="serve: length of the arrival window")
+    ap.add_argument("--chunk", type=int, default=64, help="serve: steps per batch launch chunk")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy four times (counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -200,6 +254,8 @@ def main(argv=None):
         return 0
     if args.workload == "ensemble":
         return bench_ensemble(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     from swm_amd import configs, gpu
 
     s = configs.c2_nonlinear(seed=0, N=args.n, dt=args.dt, nsteps=args.warmup + args.steps)

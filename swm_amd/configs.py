@@ -80,14 +80,17 @@ def double_gyre_tau(s, tau0):
 
 
 def fourier_modes(s, rng, n_modes=16, amp=1.0, kmax=8):
-    """Sum of n seeded Fourier modes of amplitude amp on the eta grid."""
-    x = np.arange(s.Nx)[None, :] / s.Nx
-    y = np.arange(s.Ny)[:, None] / s.Ny
+    """Sum of n seeded Fourier modes amp*cos(2 pi (kx x + ky y) + phase) on the
+    eta grid, evaluated separably as cos(a)cos(b) - sin(a)sin(b)."""
+    x = np.arange(s.Nx) / s.Nx
+    y = np.arange(s.Ny) / s.Ny
     C = np.zeros((s.Ny, s.Nx))
     for _ in range(n_modes):
         kx, ky = rng.integers(1, kmax + 1, size=2)
         ph = rng.uniform(0.0, 2.0 * math.pi)
-        C += amp * np.cos(2.0 * math.pi * (kx * x + ky * y) + ph)
+        ax = 2.0 * math.pi * kx * x + ph
+        by = 2.0 * math.pi * ky * y
+        C += amp * (np.outer(np.cos(by), np.cos(ax)) - np.outer(np.sin(by), np.sin(ax)))
     return C
 
 
@@ -168,3 +171,16 @@ def ensemble_params(seed: int = 0, n: int = 64):
 def c4_ensemble(seed: int = 0, n: int = 64, nsteps: int = 5000):
     """The batched ensemble: n independent members of uneven size."""
     return [ensemble_member(nsteps=nsteps, **p) for p in ensemble_params(seed, n)]
+
+
+def serve_member(seed: int, Nx: int, Ny: int, H0: float, Ah: float, tau0: float, tracer: bool,
+                 dx: float = 5e3, nsteps: int = 1000):
+    """One request of the serving stream: a closed nonlinear beta-plane basin
+    (walls, no islands) with the ensemble's physics and parameters."""
+    s = basin(Nx, Ny, dx, dx, variant=H.FULLY_NONLINEAR, H0=H0, dt=stable_dt(H0, dx), G=9.81, Ah=Ah, lbc=2.0,
+              f0=1e-4, beta=2e-11, drag=1e-6, lateral_mixing=True, land_fraction=0.0, seed=seed)
+    H.add_windstress(s, H.COMP_X, double_gyre_tau(s, tau0))
+    if tracer:
+        H.add_tracer(s, fourier_modes(s, s._rng), kappa_h=50.0)
+    s.Nt = nsteps
+    return H.finalize(s)

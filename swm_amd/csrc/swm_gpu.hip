@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -26,6 +27,13 @@ using swm::Fld;
 using swm::Statics;
 
 namespace {
+
+// Graph capture on one stream is invalidated by allocations and synchronous
+// copies that other host threads make while it is open.  Entry points that
+// allocate, free or copy synchronously hold this lock; the graph captures
+// hold it only while the capture is open (not while replays run).
+std::recursive_mutex g_hip_mu;
+using HipLock = std::lock_guard<std::recursive_mutex>;
 
 // Step-kernel shapes, selectable with SWM_GPU_TILE for tuning.  march = 0:
 // the LDS tile kernel with tx x ty outputs per 256-thread block.  march = 1:
@@ -526,6 +534,7 @@ int step_many(swm_ctx* c, int64_t nsteps) {
         if (steady && nsteps - done >= 6) {
             const int key = c->cur * 3 + c->ring;
             if (!c->graphs[key]) {
+                HipLock lock(g_hip_mu);
                 hipGraph_t g = nullptr;
                 HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 int rc = 0;
@@ -599,6 +608,7 @@ static thread_local std::string g_create_err;
 static int create_impl(swm_ctx* c, const swm_desc* d);
 
 int swm_gpu_create(const swm_desc* d, swm_ctx** out) {
+    HipLock lock(g_hip_mu);
     if (!out) return 1;
     *out = nullptr;
     g_create_err.clear();
@@ -705,6 +715,7 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
 
 int swm_gpu_upload_state(swm_ctx* c, const double* u, const double* v, const double* eta, const double* G_u,
                          const double* G_v, const double* G_eta, int64_t itt) {
+    HipLock lock(g_hip_mu);
     if (!c) return 1;
     if (c->batch) return fail(c, "context is attached to a batch; detach it first", 1);
     if (!u || !v || !eta) return fail(c, "u, v and eta are required", 1);
@@ -718,17 +729,26 @@ int swm_gpu_upload_state(swm_ctx* c, const double* u, const double* v, const dou
     if ((rc = upload_grid(c, plane_ptr(c, swm::PL_U), u)) || (rc = upload_grid(c, plane_ptr(c, swm::PL_V), v)) ||
         (rc = upload_grid(c, plane_ptr(c, swm::PL_E), eta)))
         return rc;
-    std::vector<double> zero(n, 0.0);
+    const size_t pbytes = (size_t)c->plane * c->esz;
     for (int l = 0; l < 3; ++l) {
-        if ((rc = upload_grid(c, plane_ptr(c, swm::PL_GU + l), G_u ? G_u + l * n : zero.data()))) return rc;
-        if ((rc = upload_grid(c, plane_ptr(c, swm::PL_GV + l), G_v ? G_v + l * n : zero.data()))) return rc;
-        if ((rc = upload_grid(c, plane_ptr(c, swm::PL_GE + l), G_eta ? G_eta + l * n : zero.data()))) return rc;
+        const double* src[3] = {G_u ? G_u + l * n : nullptr, G_v ? G_v + l * n : nullptr,
+                                G_eta ? G_eta + l * n : nullptr};
+        const int pl[3] = {swm::PL_GU + l, swm::PL_GV + l, swm::PL_GE + l};
+        for (int f = 0; f < 3; ++f) {
+            if (src[f]) {
+                if ((rc = upload_grid(c, plane_ptr(c, pl[f]), src[f]))) return rc;
+            } else {   // no history given: zero tendencies (0.0 is all-zero bits)
+                HIPCHK(c, hipMemsetAsync(plane_ptr(c, pl[f]), 0, pbytes, c->stream));
+            }
+        }
     }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     c->itt = itt;
     return 0;
 }
 
 int swm_gpu_add_tracer(swm_ctx* c, const swm_tracer_desc* t, int32_t* tracer_id) {
+    HipLock lock(g_hip_mu);
     if (!c || !t) return 1;
     if (c->batch) return fail(c, "context is attached to a batch; detach it first", 1);
     if ((int)c->trc.size() >= kMaxTracers) return fail(c, "at most 2 tracers are fused into the step kernel", 4);
@@ -745,10 +765,15 @@ int swm_gpu_add_tracer(swm_ctx* c, const swm_tracer_desc* t, int32_t* tracer_id)
     // the tracer's planes follow the context's current ping-pong / ring roles
     if ((rc = upload_grid(c, plane_ptr(c, base + c->cur), t->CH))) return rc;
     if ((rc = upload_grid(c, plane_ptr(c, base + 5), t->C))) return rc;
-    std::vector<double> zero(n, 0.0);
-    for (int l = 0; l < 3; ++l)
-        if ((rc = upload_grid(c, plane_ptr(c, base + 2 + (c->ring + l) % 3), t->G_CH ? t->G_CH + l * n : zero.data())))
-            return rc;
+    for (int l = 0; l < 3; ++l) {
+        void* dst = plane_ptr(c, base + 2 + (c->ring + l) % 3);
+        if (t->G_CH) {
+            if ((rc = upload_grid(c, dst, t->G_CH + l * n))) return rc;
+        } else {
+            HIPCHK(c, hipMemsetAsync(dst, 0, (size_t)c->plane * c->esz, c->stream));
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     Tracer tr;
     tr.kappa = t->kappa_h;
     tr.cbuf_valid = true;
@@ -781,6 +806,7 @@ int swm_gpu_step(swm_ctx* c, int64_t nsteps) {
 
 int64_t swm_gpu_itt(const swm_ctx* c) {
     if (!c) return -1;
+    HipLock lock(g_hip_mu);
     if (c->batch && member_refresh(const_cast<swm_ctx*>(c))) return -1;
     return c->itt;
 }
@@ -793,6 +819,7 @@ int swm_gpu_last_timing(const swm_ctx* c, double* total_ms, double* step_kernel_
 }
 
 int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_out) {
+    HipLock lock(g_hip_mu);
     if (!c || !name || !host_out) return 1;
     if (int rc = member_refresh(c)) return rc;
     std::string s(name);
@@ -983,6 +1010,7 @@ extern "C" {
 
 int swm_gpu_sor(swm_ctx* c, const double* B, double* chi, double epsilon, int32_t first_guess, int64_t* iters,
                 double* anorm) {
+    HipLock lock(g_hip_mu);
     if (!c || !B || !chi || !iters || !anorm) return 1;
     const int mx = std::max(c->Nx, c->Ny);
     const int64_t max_count = first_guess ? (int64_t)lrintf(1000.f * (float)mx) : (int64_t)lrintf(10.f * (float)mx);
@@ -991,11 +1019,13 @@ int swm_gpu_sor(swm_ctx* c, const double* B, double* chi, double epsilon, int32_
 
 int swm_gpu_sor_fixed(swm_ctx* c, const double* B, double* chi, double omega, double rel_tol, int64_t max_sweeps,
                       int32_t fp32_sweeps, int64_t* iters, double* rel_res) {
+    HipLock lock(g_hip_mu);
     if (!c || !B || !chi || !iters || !rel_res) return 1;
     return sor_common(c, B, chi, 0, omega, rel_tol, max_sweeps, fp32_sweeps, iters, rel_res);
 }
 
 int swm_gpu_divergence(swm_ctx* c, double* host_div) {
+    HipLock lock(g_hip_mu);
     if (!c || !host_div) return 1;
     if (int rc = member_refresh(c)) return rc;
     const int n = c->Nx * c->Ny;
@@ -1018,6 +1048,7 @@ int swm_gpu_divergence(swm_ctx* c, double* host_div) {
 }
 
 int swm_gpu_sums(swm_ctx* c, int32_t tracer_id, double* out) {
+    HipLock lock(g_hip_mu);
     if (!c || !out) return 1;
     if (int rc = member_refresh(c)) return rc;
     if (tracer_id >= (int32_t)c->trc.size()) return fail(c, "tracer id out of range", 1);
@@ -1058,6 +1089,7 @@ int swm_gpu_sums(swm_ctx* c, int32_t tracer_id, double* out) {
 const char* swm_gpu_last_error(const swm_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
 
 void swm_gpu_destroy(swm_ctx* c) {
+    HipLock lock(g_hip_mu);
     if (!c) return;
     if (c->batch) (void)swm_gpu_batch_detach(c->batch, c);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -1228,6 +1260,7 @@ int member_refresh(swm_ctx* c) {
 extern "C" {
 
 int swm_gpu_batch_create(swm_batch** out) {
+    HipLock lock(g_hip_mu);
     if (!out) return 1;
     *out = nullptr;
     swm_batch* b = new swm_batch();
@@ -1242,6 +1275,7 @@ int swm_gpu_batch_create(swm_batch** out) {
 }
 
 int swm_gpu_batch_attach(swm_batch* b, swm_ctx* c, int64_t nsteps) {
+    HipLock lock(g_hip_mu);
     if (!b || !c) return 1;
     if (c->batch) return bfail(b, "context is already attached to a batch", 1);
     if (nsteps < 0) return bfail(b, "nsteps must be >= 0", 1);
@@ -1268,6 +1302,7 @@ int swm_gpu_batch_attach(swm_batch* b, swm_ctx* c, int64_t nsteps) {
 }
 
 int swm_gpu_batch_detach(swm_batch* b, swm_ctx* c) {
+    HipLock lock(g_hip_mu);
     if (!b || !c) return 1;
     if (c->batch != b) return bfail(b, "context is not attached to this batch", 1);
     if (int rc = batch_pull(b)) return rc;
@@ -1281,6 +1316,7 @@ int swm_gpu_batch_detach(swm_batch* b, swm_ctx* c) {
 }
 
 int swm_gpu_batch_set_steps(swm_batch* b, swm_ctx* c, int64_t nsteps) {
+    HipLock lock(g_hip_mu);
     if (!b || !c) return 1;
     if (c->batch != b) return bfail(b, "context is not attached to this batch", 1);
     if (nsteps < 0) return bfail(b, "nsteps must be >= 0", 1);
@@ -1292,6 +1328,7 @@ int swm_gpu_batch_set_steps(swm_batch* b, swm_ctx* c, int64_t nsteps) {
 }
 
 int swm_gpu_batch_remaining(swm_batch* b, const swm_ctx* c, int64_t* remaining) {
+    HipLock lock(g_hip_mu);
     if (!b || !c || !remaining) return 1;
     if (c->batch != b) return bfail(b, "context is not attached to this batch", 1);
     if (int rc = batch_pull(b)) return rc;
@@ -1304,6 +1341,7 @@ int swm_gpu_batch_step(swm_batch* b, int64_t nsteps, int32_t* active) {
     if (!b) return 1;
     if (nsteps < 0) return bfail(b, "nsteps must be >= 0", 1);
     if (b->dirty) {
+        HipLock lock(g_hip_mu);
         if (int rc = batch_pull(b)) return rc;
         int rc = b->fp32 == 1 ? batch_build_t<float>(b) : batch_build_t<double>(b);
         if (rc) return rc;
@@ -1311,6 +1349,7 @@ int swm_gpu_batch_step(swm_batch* b, int64_t nsteps, int32_t* active) {
     b->last_ms = 0.0;
     if (!b->mem.empty() && nsteps > 0) {
         if (!b->graph) {
+            HipLock lock(g_hip_mu);
             hipGraph_t g = nullptr;
             BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             batch_launch(b);
@@ -1337,6 +1376,7 @@ int swm_gpu_batch_step(swm_batch* b, int64_t nsteps, int32_t* active) {
         b->on_device = true;
     }
     if (active) {
+        HipLock lock(g_hip_mu);
         if (int rc = batch_pull(b)) return rc;
         int32_t na = 0;
         for (const auto& h : b->hstep) na += h.remaining > 0;
@@ -1354,6 +1394,7 @@ int swm_gpu_batch_last_timing(const swm_batch* b, double* total_ms) {
 const char* swm_gpu_batch_last_error(const swm_batch* b) { return b ? b->err.c_str() : g_create_err.c_str(); }
 
 void swm_gpu_batch_destroy(swm_batch* b) {
+    HipLock lock(g_hip_mu);
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     (void)batch_pull(b);
