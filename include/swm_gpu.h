@@ -146,6 +146,17 @@ int swm_gpu_sor(swm_ctx *c, const double *B, double *chi, double epsilon, int32_
 int swm_gpu_sor_fixed(swm_ctx *c, const double *B, double *chi, double omega, double rel_tol,
                       int64_t max_sweeps, int32_t fp32_sweeps, int64_t *iters, double *rel_res);
 
+/* computeVelocityPotential (src/calc_lib.f90:500-525) without host copies:
+ * B = computeDivergence of the current device velocity (calc_lib.f90:607-622)
+ * in the sweep precision, then fixed-omega red-black SOR as in
+ * swm_gpu_sor_fixed (FP64 ||B|| and convergence test also for FP32 sweeps),
+ * warm-started from the previous solution of the same sweep precision kept
+ * on the device (zero on the first call or when reset != 0).  The solution
+ * stays on the device; swm_gpu_sync(c, "CHI", -1, out) copies the last one.
+ * Same return convention as swm_gpu_sor_fixed. */
+int swm_gpu_velocity_potential(swm_ctx *c, double omega, double rel_tol, int64_t max_sweeps, int32_t fp32_sweeps,
+                               int32_t reset, int64_t *iters, double *rel_res);
+
 /* computeDivergence(u, v, div, u_grid, v_grid) of the current device velocity
  * (src/calc_lib.f90:607-622), the right-hand side computeVelocityPotential
  * hands to the SOR (src/calc_lib.f90:500-525).  Copies div to the host. */

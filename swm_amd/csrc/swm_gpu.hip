@@ -39,10 +39,13 @@ using HipLock = std::lock_guard<std::recursive_mutex>;
 // the LDS tile kernel with tx x ty outputs per 256-thread block.  march = 1:
 // the marching-wavefront kernel, four wavefronts per block, each owning 62
 // columns and ty rows; ty = 0 picks the smallest ty for which every
-// wavefront of the grid is resident at once (one round, no tail).
+// wavefront of the grid is resident at once (one round, no tail).  march = 2:
+// the persistent tile kernel (tx x ty tiles, each block a run of tiles sized
+// so that all blocks are resident at once).
 struct TileShape { int march, tx, ty; };
 constexpr TileShape kTiles[] = {{0, 64, 4}, {0, 32, 8}, {0, 64, 8}, {0, 128, 4}, {0, 32, 16},
-                                {1, 62, 0}, {1, 62, 8}, {1, 62, 16}, {1, 62, 4}};
+                                {1, 62, 0}, {1, 62, 8}, {1, 62, 16}, {1, 62, 4},
+                                {2, 64, 4}, {2, 32, 8}};
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 constexpr int kMaxTracers = 2;
 
@@ -96,9 +99,13 @@ struct swm_ctx {
     bool use_graph = true;
     int num_cu = 0;              // compute units of the device
     double* sums_buf = nullptr;  // conservation-sum partials and result
+    double* norm_buf = nullptr;  // ||B|| partials and result of the device-resident SOR
+    bool chi_valid[2] = {};      // sor_buf[w][0] holds a solution to warm-start from
+    int chi_last = -1;           // precision slot of the last solution
     double* ce64 = nullptr;      // cos(lat_eta) rows in FP64
     std::vector<double> ce_host;
     long march_slots = 0;        // resident wavefronts of the marching kernel (0: not queried)
+    long persist_slots[2] = {};  // resident blocks of the persistent tile kernels (0: not queried)
     hipGraphExec_t graphs[6] = {};   // steady-state 6-step cycles, keyed by (cur, ring)
     // diagnostics scratch
     void* diag[14] = {};
@@ -400,6 +407,26 @@ int march_blocks_per_cu(swm_ctx* c) {
     return e == hipSuccess ? nb : 0;
 }
 
+template <typename T, int TX, int TY>
+int persist_blocks_per_cu(swm_ctx* c) {
+    const int var = c->d.variant;
+    const int nt = (int)c->trc.size();
+    int nb = 0;
+    hipError_t e = hipSuccess;
+#define SWM_OCC(V, N) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, swm::step_kernel_p<T, V, N, TX, TY>, 256, 0)
+    if (var == SWM_FULLY_NONLINEAR) {
+        if (nt == 0) SWM_OCC(swm::VAR_FN, 0);
+        else if (nt == 1) SWM_OCC(swm::VAR_FN, 1);
+        else SWM_OCC(swm::VAR_FN, 2);
+    } else {
+        if (nt == 0) SWM_OCC(swm::VAR_SOR, 0);
+        else if (nt == 1) SWM_OCC(swm::VAR_SOR, 1);
+        else SWM_OCC(swm::VAR_SOR, 2);
+    }
+#undef SWM_OCC
+    return e == hipSuccess ? nb : 0;
+}
+
 // Rows per wavefront of the marching kernel: the smallest count (at least 4)
 // whose grid fits the device's resident wavefront slots in one round.
 int march_rows(swm_ctx* c, int ntx) {
@@ -417,11 +444,22 @@ Dyn<T> make_dyn(swm_ctx* c, int tile) {
     d.plane = c->plane;
     const TileShape ts = kTiles[tile];
     d.ntx = (c->Nx + ts.tx - 1) / ts.tx;
-    if (ts.march && !ts.ty && c->march_slots <= 0)
+    if (ts.march == 1 && !ts.ty && c->march_slots <= 0)
         c->march_slots = (long)march_blocks_per_cu<T>(c) * 4 * c->num_cu;
-    d.ty = ts.ty ? ts.ty : march_rows(c, d.ntx);
-    const int nty = (c->Ny + d.ty - 1) / d.ty;
-    d.nblk = d.ntx * (ts.march ? (nty + 3) / 4 : nty);
+    if (ts.march == 2) {   // persistent tiles: nblk counts tiles, the grid is nblk / tpb blocks
+        d.ty = ts.ty;
+        d.nblk = d.ntx * ((c->Ny + ts.ty - 1) / ts.ty);
+        const int k = ts.tx == 64 ? 0 : 1;
+        if (c->persist_slots[k] <= 0) {
+            const int nb = ts.tx == 64 ? persist_blocks_per_cu<T, 64, 4>(c) : persist_blocks_per_cu<T, 32, 8>(c);
+            c->persist_slots[k] = (long)std::max(1, nb) * std::max(1, c->num_cu);
+        }
+        d.tpb = (int)std::max<long>(1, (d.nblk + c->persist_slots[k] - 1) / c->persist_slots[k]);
+    } else {
+        d.ty = ts.ty ? ts.ty : march_rows(c, d.ntx);
+        const int nty = (c->Ny + d.ty - 1) / d.ty;
+        d.nblk = d.ntx * (ts.march ? (nty + 3) / 4 : nty);
+    }
     d.cur = c->cur;
     d.ring = c->ring;
     d.tstep = 1;
@@ -468,6 +506,24 @@ void launch_march(swm_ctx* c, const Dyn<T>& d) {
 #undef SWM_LAUNCH
 }
 
+template <typename T, int TX, int TY>
+void launch_persist(swm_ctx* c, const Dyn<T>& d) {
+    const int var = c->d.variant;
+    const int nt = (int)c->trc.size();
+    dim3 grid((d.nblk + d.tpb - 1) / d.tpb), block(256);
+#define SWM_LAUNCH(V, N) hipLaunchKernelGGL((swm::step_kernel_p<T, V, N, TX, TY>), grid, block, 0, c->stream, d)
+    if (var == SWM_FULLY_NONLINEAR) {
+        if (nt == 0) SWM_LAUNCH(swm::VAR_FN, 0);
+        else if (nt == 1) SWM_LAUNCH(swm::VAR_FN, 1);
+        else SWM_LAUNCH(swm::VAR_FN, 2);
+    } else {
+        if (nt == 0) SWM_LAUNCH(swm::VAR_SOR, 0);
+        else if (nt == 1) SWM_LAUNCH(swm::VAR_SOR, 1);
+        else SWM_LAUNCH(swm::VAR_SOR, 2);
+    }
+#undef SWM_LAUNCH
+}
+
 template <typename T, bool DIAG>
 int launch_step_t(swm_ctx* c, const Dyn<T>& d, int tile) {
     if (DIAG) {
@@ -479,6 +535,8 @@ int launch_step_t(swm_ctx* c, const Dyn<T>& d, int tile) {
         case 3: launch_tile<T, 128, 4, false>(c, d); break;
         case 4: launch_tile<T, 32, 16, false>(c, d); break;
         case 5: case 6: case 7: case 8: launch_march<T>(c, d); break;
+        case 9: launch_persist<T, 64, 4>(c, d); break;
+        case 10: launch_persist<T, 32, 8>(c, d); break;
         default: launch_tile<T, 64, 4, false>(c, d); break;
         }
     }
@@ -783,6 +841,7 @@ int swm_gpu_add_tracer(swm_ctx* c, const swm_tracer_desc* t, int32_t* tracer_id)
         return rc;
     c->trc.push_back(tr);
     c->march_slots = 0;   // the kernel instance (and its occupancy) depends on the tracer count
+    c->persist_slots[0] = c->persist_slots[1] = 0;
     if ((rc = write_statics(c))) return rc;
     if (tracer_id) *tracer_id = nt;
     return 0;
@@ -824,6 +883,20 @@ int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_o
     if (int rc = member_refresh(c)) return rc;
     std::string s(name);
     for (auto& ch : s) ch = (char)toupper((unsigned char)ch);
+    if (s == "CHI") {   // the last device-resident SOR solution
+        if (c->chi_last < 0) return fail(c, "no velocity potential has been computed", 1);
+        const int w = c->chi_last;
+        const size_t n = (size_t)c->Nx * c->Ny;
+        if (w == 1) {
+            std::vector<float> f(n);
+            HIPCHK(c, hipMemcpy2D(f.data(), c->Nx * 4, c->sor_buf[1][0], c->P * 4, c->Nx * 4, c->Ny, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < n; ++k) host_out[k] = f[k];
+        } else {
+            HIPCHK(c, hipMemcpy2D(host_out, c->Nx * 8, c->sor_buf[0][0], c->P * 8, c->Nx * 8, c->Ny,
+                                  hipMemcpyDeviceToHost));
+        }
+        return 0;
+    }
     const int n0 = c->cur;
     const int r1 = (c->ring + 1) % 3, r0 = c->ring;
     const void* src = nullptr;
@@ -1001,6 +1074,57 @@ int sor_common(swm_ctx* c, const double* B, double* chi, int ref_mode, double om
     }
     *iters = it;
     *norm = nrm;
+    c->chi_valid[w] = true;
+    c->chi_last = w;
+    return host_done ? 0 : 1;
+}
+
+template <typename TF, typename TS>
+int velpot_t(swm_ctx* c, double omega, double rel_tol, int64_t max_sweeps, bool reset, int64_t* iters,
+             double* rel_res) {
+    const int Nx = c->Nx, Ny = c->Ny;
+    const int half = (Nx + 1) / 2;
+    const int nblk = (int)(((long)Ny * half + 255) / 256);
+    const bool f32 = std::is_same<TS, float>::value;
+    int rc = sor_prepare(c, f32, nblk);
+    if (rc) return rc;
+    const int w = f32 ? 1 : 0;
+    void* dchi = c->sor_buf[w][0];
+    TS* dB = (TS*)c->sor_buf[w][1];
+    const long n = (long)Nx * Ny;
+    if (!c->norm_buf && !(c->norm_buf = (double*)dalloc(c, sizeof(double) * 1025)))
+        return fail(c, "out of device memory", 3);
+    // right-hand side: computeDivergence of the current velocity (calc_lib.f90:500-525)
+    hipLaunchKernelGGL((swm::divergence_kernel<TF, TS>), dim3((n + 255) / 256), dim3(256), 0, c->stream, Nx, Ny, c->P,
+                       (const TF*)plane_ptr(c, swm::PL_U + c->cur), (const TF*)plane_ptr(c, swm::PL_V + c->cur),
+                       c->mask, (const TF*)c->dza, (const TF*)c->tab[T_CV], (const TF*)c->tab[T_CE],
+                       (TF)(c->d.A * c->d.dTheta), (TF)c->d.lbc, dB);
+    HIPCHK(c, hipGetLastError());
+    if (reset || !c->chi_valid[w])
+        HIPCHK(c, hipMemsetAsync(dchi, 0, (size_t)c->P * Ny * sizeof(TS), c->stream));
+    // ||B|| over ocean eta points in FP64 (fixed order)
+    const int nb = (int)std::min<long>(1024, (n + 255) / 256);
+    hipLaunchKernelGGL(swm::sq_norm_kernel<TS>, dim3(nb), dim3(256), 0, c->stream, Nx, Ny, c->P, (const TS*)dB, c->mask,
+                       c->norm_buf);
+    hipLaunchKernelGGL(swm::sq_norm_final, dim3(1), dim3(256), 0, c->stream, (const double*)c->norm_buf, nb,
+                       c->norm_buf + 1024);
+    HIPCHK(c, hipGetLastError());
+    double bnorm = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&bnorm, c->norm_buf + 1024, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->sor_done, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->sor_iters, 0, sizeof(long long), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int host_done = 0;
+    if ((rc = sor_sweeps<TS>(c, dchi, dB, c->sor_ftab[w], false, omega, rel_tol, bnorm, max_sweeps, nblk, &host_done)))
+        return rc;
+    long long it = 0;
+    double nrm = 0.0;
+    HIPCHK(c, hipMemcpy(&it, c->sor_iters, sizeof(it), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(&nrm, c->sor_norm, sizeof(nrm), hipMemcpyDeviceToHost));
+    *iters = it;
+    *rel_res = nrm;
+    c->chi_valid[w] = true;
+    c->chi_last = w;
     return host_done ? 0 : 1;
 }
 
@@ -1022,6 +1146,18 @@ int swm_gpu_sor_fixed(swm_ctx* c, const double* B, double* chi, double omega, do
     HipLock lock(g_hip_mu);
     if (!c || !B || !chi || !iters || !rel_res) return 1;
     return sor_common(c, B, chi, 0, omega, rel_tol, max_sweeps, fp32_sweeps, iters, rel_res);
+}
+
+int swm_gpu_velocity_potential(swm_ctx* c, double omega, double rel_tol, int64_t max_sweeps, int32_t fp32_sweeps,
+                               int32_t reset, int64_t* iters, double* rel_res) {
+    if (!c || !iters || !rel_res) return 1;
+    HipLock lock(g_hip_mu);
+    if (int rc = member_refresh(c)) return rc;
+    if (!(omega > 0.0 && omega < 2.0)) return fail(c, "omega must lie in (0, 2)", 1);
+    if (max_sweeps < 1) return fail(c, "max_sweeps must be >= 1", 1);
+    if (c->fp32) return velpot_t<float, float>(c, omega, rel_tol, max_sweeps, reset != 0, iters, rel_res);
+    if (fp32_sweeps) return velpot_t<double, float>(c, omega, rel_tol, max_sweeps, reset != 0, iters, rel_res);
+    return velpot_t<double, double>(c, omega, rel_tol, max_sweeps, reset != 0, iters, rel_res);
 }
 
 int swm_gpu_divergence(swm_ctx* c, double* host_div) {

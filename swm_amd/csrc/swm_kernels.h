@@ -79,6 +79,7 @@ struct Dyn {
     long long plane;     // elements per plane (Ny * P)
     int ntx, nblk;
     int ty;              // rows per wavefront (marching kernel)
+    int tpb;             // tiles per block (persistent tile kernel)
     int cur, ring;       // ping-pong level of N0; ring slot of G(1)
     int tstep;           // min(3, itt)
     int cbuf;            // bit n: tracer n reads its C buffer (first step after hand-over)
@@ -174,21 +175,155 @@ __device__ __forceinline__ T trc_adv_r(bool neg, int nummask, int den, const T* 
 }
 
 // LDS of one tile: the state with a one-cell halo, the derived fields and the
-// per-row coefficient tables of its rows.  NTR is the largest tracer count the
+// per-row coefficient tables of its rows.  eta shares its array with Pot (each
+// cell reads its own eta before writing its Pot), and the mass fluxes MU, MV
+// are recomputed from D, u, v where they are needed, which keeps the FP64
+// tile under 32 KiB (five blocks per CU).  NTR is the largest tracer count the
 // kernel handles.
 template <typename T, int NTR, int E, int EH>
 struct TileLds {
-    T u[E], v[E], eta[E], D[E], Pot[E], Ed[E], MU[E], MV[E], pll[E], plt[E];
+    T u[E], v[E], D[E], Pot[E], Ed[E], pll[E], plt[E];   // Pot holds eta until phase 2 replaces it
     T CH[NTR][E], C[NTR][E];
     // per-row tables of the EH tile rows (cu also of the row above them)
     T za[EH][2], cu[EH + 1], cH[EH], fH[EH], pltc[EH][12], pllc[EH][12];
     uint8_t m[E];
 };
 
-// One 256-thread block advances tile t (TX x TY cells) of the domain in d by
-// one step.  L is a TileLds with room for at least NTRC tracers.
-template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, typename L>
-__device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) {
+// Registers of one tile between its loads and their LDS stores: the state with
+// halo (NE cells per thread), the per-row tables and the AB history of the
+// thread's NO output cells.
+template <typename T, int NE, int NO, int NTR>
+struct TileRegs {
+    int m[NE];
+    T u[NE], v[NE], e[NE], Ho[NE], Hl[NE], ch[NTR][NE], cb[NTR][NE];
+    T tz0, tz1, tcu, tcH, tfH, tcu0, tpc;
+    T g[NO][6];
+    T gt[NO][NTR][2];
+    T eo[NO];    // eta at the outputs
+};
+
+template <int TX, int TY>
+struct TileShapeC {
+    static constexpr int NT = 256, EW = TX + 2, EH = TY + 2, E = EW * EH;
+    static constexpr int NO = (TX * TY + NT - 1) / NT, NE = (E + NT - 1) / NT;
+};
+
+// Issue every global load of tile t: the state with its halo, the per-row
+// tables and the AB history of the outputs.
+template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, typename RG>
+__device__ __forceinline__ void tile_load(const Dyn<T>& d, const int t, RG& R) {
+    constexpr int NT = 256;
+    constexpr int EW = TX + 2;
+    constexpr int EH = TY + 2;
+    constexpr int E = EW * EH;
+    constexpr int NO = (TX * TY + NT - 1) / NT;
+    constexpr int NTR = NTRC > 0 ? NTRC : 1;
+    const Statics<T>& a = *d.s;
+    const int tid = threadIdx.x;
+    const int i0 = (t % d.ntx) * TX;
+    const int j0 = (t / d.ntx) * TY;
+    const int Nx = a.Nx, Ny = a.Ny, P = a.P;
+    const bool FN = (VAR == VAR_FN);
+    // planes of this step's roles
+    T* const st = d.state;
+    const long long pl = d.plane;
+    const int r0 = d.ring, r1 = d.ring == 2 ? 0 : d.ring + 1, r2 = d.ring == 0 ? 2 : d.ring - 1;
+    const T* const su = st + (PL_U + d.cur) * pl;
+    const T* const sv = st + (PL_V + d.cur) * pl;
+    const T* const se = st + (PL_E + d.cur) * pl;
+    const T* const Gu1 = st + (PL_GU + r0) * pl;
+    const T* const Gu2 = st + (PL_GU + r1) * pl;
+    const T* const Gv1 = st + (PL_GV + r0) * pl;
+    const T* const Gv2 = st + (PL_GV + r1) * pl;
+    const T* const Ge1 = st + (PL_GE + r0) * pl;
+    const T* const Ge2 = st + (PL_GE + r1) * pl;
+
+    // ---- prefetch this thread's AB history (independent of the LDS phases)
+    auto& pg = R.g;
+    auto& pt = R.gt;
+    if (!DIAG) {
+#pragma unroll
+        for (int q = 0; q < NO; ++q) {
+            const int o = tid + q * NT;
+            const int i = i0 + o % TX, j = j0 + o / TX;
+            if (o < TX * TY && i < Nx && j < Ny) {
+                const size_t off = (size_t)j * P + i;
+                pg[q][0] = Gu1[off]; pg[q][1] = Gu2[off];
+                pg[q][2] = Gv1[off]; pg[q][3] = Gv2[off];
+                pg[q][4] = Ge1[off]; pg[q][5] = Ge2[off];
+                R.eo[q] = se[off];
+#pragma unroll
+                for (int n = 0; n < NTRC; ++n) {
+                    const T* tb = st + (PL_TRC + PL_PER_TRC * n + 2) * pl;
+                    pt[q][n][0] = tb[r0 * pl + off];
+                    pt[q][n][1] = tb[r1 * pl + off];
+                }
+            }
+        }
+    }
+
+    // ---- phase 1: state with a one-cell halo (periodic indices) into LDS.
+    // Every global load of the block is issued before the first LDS store, so
+    // one memory latency covers the whole tile and its per-row tables.
+    constexpr int NE = (E + NT - 1) / NT;
+    auto& rm = R.m;
+    auto& ru = R.u;
+    auto& rv = R.v;
+    auto& re = R.e;
+    auto& rHo = R.Ho;
+    auto& rHl = R.Hl;
+    auto& rch = R.ch;
+    auto& rcb = R.cb;
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+        const int e = tid + q * NT;
+        if (e < E) {
+            const int lx = e % EW, ly = e / EW;
+            const int gi = wrap(i0 + lx - 1, Nx), gj = wrap(j0 + ly - 1, Ny);
+            const size_t off = (size_t)gj * P + gi;
+            rm[q] = a.mask[off];
+            ru[q] = su[off];
+            rv[q] = sv[off];
+            re[q] = se[off];
+            // H_eta at an ocean (rHo) and a land (rHl) point of this cell's row
+            rHo[q] = a.He.full ? a.He.full[off] : a.He.rows[2 * gj];
+            rHl[q] = a.He.full ? rHo[q] : a.He.rows[2 * gj + 1];
+#pragma unroll
+            for (int n = 0; n < NTRC; ++n) {
+                const T* tb = st + (PL_TRC + PL_PER_TRC * n) * pl;
+                rch[n][q] = tb[d.cur * pl + off];
+                rcb[n][q] = ((d.cbuf >> n) & 1) ? tb[5 * pl + off] : T(0);
+            }
+        }
+    }
+    // per-row tables of rows j0-1 .. j0+TY for phase 2 (cu also of row j0-2)
+    T& tz0 = R.tz0;
+    T& tz1 = R.tz1;
+    T& tcu = R.tcu;
+    T& tcH = R.tcH;
+    T& tfH = R.tfH;
+    T& tcu0 = R.tcu0;
+    T& tpc = R.tpc;
+    if (tid < EH) {
+        const int gj = wrap(j0 + tid - 1, Ny);
+        tz0 = a.za[2 * gj];
+        tz1 = a.za[2 * gj + 1];
+        tcu = a.cu[gj];
+        tcH = a.cH[gj];
+        tfH = a.fH[gj];
+    }
+    if (tid == NT - 1) tcu0 = a.cu[wrap(j0 - 2, Ny)];
+    const bool has_pc = a.mix && tid < EH * 24;
+    if (has_pc) {
+        const int r = tid / 24, c = tid % 24;
+        const int gj = wrap(j0 + r - 1, Ny);
+        tpc = c < 12 ? a.pltc[gj * 12 + c] : a.pllc[gj * 12 + c - 12];
+    }
+}
+
+// Derive D and C from the loaded state and write the tile to LDS.
+template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, typename RG, typename L>
+__device__ __forceinline__ void tile_stage(const Dyn<T>& d, const int t, const RG& R, L& lds) {
     constexpr int NT = 256;
     constexpr int EW = TX + 2;
     constexpr int EH = TY + 2;
@@ -197,12 +332,10 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
     constexpr int NTR = NTRC > 0 ? NTRC : 1;
     T* const s_u = lds.u;
     T* const s_v = lds.v;
-    T* const s_eta = lds.eta;
+    T* const s_eta = lds.Pot;
     T* const s_D = lds.D;
     T* const s_Pot = lds.Pot;
     T* const s_Ed = lds.Ed;
-    T* const s_MU = lds.MU;
-    T* const s_MV = lds.MV;
     T* const s_pll = lds.pll;
     T* const s_plt = lds.plt;
     auto& s_CH = lds.CH;
@@ -229,74 +362,17 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
     const T* const Ge1 = st + (PL_GE + r0) * pl;
     const T* const Ge2 = st + (PL_GE + r1) * pl;
 
-    // ---- prefetch this thread's AB history (independent of the LDS phases)
-    T pg[NO][6];
-    T pt[NO][NTR][2];
-    if (!DIAG) {
-#pragma unroll
-        for (int q = 0; q < NO; ++q) {
-            const int o = tid + q * NT;
-            const int i = i0 + o % TX, j = j0 + o / TX;
-            if (o < TX * TY && i < Nx && j < Ny) {
-                const size_t off = (size_t)j * P + i;
-                pg[q][0] = Gu1[off]; pg[q][1] = Gu2[off];
-                pg[q][2] = Gv1[off]; pg[q][3] = Gv2[off];
-                pg[q][4] = Ge1[off]; pg[q][5] = Ge2[off];
-#pragma unroll
-                for (int n = 0; n < NTRC; ++n) {
-                    const T* tb = st + (PL_TRC + PL_PER_TRC * n + 2) * pl;
-                    pt[q][n][0] = tb[r0 * pl + off];
-                    pt[q][n][1] = tb[r1 * pl + off];
-                }
-            }
-        }
-    }
-
-    // ---- phase 1: state with a one-cell halo (periodic indices) into LDS.
-    // Every global load of the block is issued before the first LDS store, so
-    // one memory latency covers the whole tile and its per-row tables.
     constexpr int NE = (E + NT - 1) / NT;
-    int rm[NE];
-    T ru[NE], rv[NE], re[NE], rHo[NE], rHl[NE], rch[NTR][NE], rcb[NTR][NE];
-#pragma unroll
-    for (int q = 0; q < NE; ++q) {
-        const int e = tid + q * NT;
-        if (e < E) {
-            const int lx = e % EW, ly = e / EW;
-            const int gi = wrap(i0 + lx - 1, Nx), gj = wrap(j0 + ly - 1, Ny);
-            const size_t off = (size_t)gj * P + gi;
-            rm[q] = a.mask[off];
-            ru[q] = su[off];
-            rv[q] = sv[off];
-            re[q] = se[off];
-            // H_eta at an ocean (rHo) and a land (rHl) point of this cell's row
-            rHo[q] = a.He.full ? a.He.full[off] : a.He.rows[2 * gj];
-            rHl[q] = a.He.full ? rHo[q] : a.He.rows[2 * gj + 1];
-#pragma unroll
-            for (int n = 0; n < NTRC; ++n) {
-                const T* tb = st + (PL_TRC + PL_PER_TRC * n) * pl;
-                rch[n][q] = tb[d.cur * pl + off];
-                rcb[n][q] = ((d.cbuf >> n) & 1) ? tb[5 * pl + off] : T(0);
-            }
-        }
-    }
-    // per-row tables of rows j0-1 .. j0+TY for phase 2 (cu also of row j0-2)
-    T tz0 = T(0), tz1 = T(0), tcu = T(0), tcH = T(0), tfH = T(0), tcu0 = T(0), tpc = T(0);
-    if (tid < EH) {
-        const int gj = wrap(j0 + tid - 1, Ny);
-        tz0 = a.za[2 * gj];
-        tz1 = a.za[2 * gj + 1];
-        tcu = a.cu[gj];
-        tcH = a.cH[gj];
-        tfH = a.fH[gj];
-    }
-    if (tid == NT - 1) tcu0 = a.cu[wrap(j0 - 2, Ny)];
+    const auto& rm = R.m;
+    const auto& ru = R.u;
+    const auto& rv = R.v;
+    const auto& re = R.e;
+    const auto& rHo = R.Ho;
+    const auto& rHl = R.Hl;
+    const auto& rch = R.ch;
+    const auto& rcb = R.cb;
+    const T tz0 = R.tz0, tz1 = R.tz1, tcu = R.tcu, tcH = R.tcH, tfH = R.tfH, tcu0 = R.tcu0, tpc = R.tpc;
     const bool has_pc = a.mix && tid < EH * 24;
-    if (has_pc) {
-        const int r = tid / 24, c = tid % 24;
-        const int gj = wrap(j0 + r - 1, Ny);
-        tpc = c < 12 ? a.pltc[gj * 12 + c] : a.pllc[gj * 12 + c - 12];
-    }
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
         const int e = tid + q * NT;
@@ -340,7 +416,50 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
             if (c < 12) lds.pltc[r][c] = a.pltc[gj * 12 + c];
             else lds.pllc[r][c - 12] = a.pllc[gj * 12 + c - 12];
         }
-    __syncthreads();
+}
+
+// Phases 2 and 3 of tile t, after its LDS stage and a barrier: derived fields,
+// tendencies, AB integration and stores.  pg/pt: the outputs' AB history.
+template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, typename L, typename G, typename GT, typename GE>
+__device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& lds, const G& pg, const GT& pt,
+                                             const GE& pe) {
+    constexpr int NT = 256;
+    constexpr int EW = TX + 2;
+    constexpr int EH = TY + 2;
+    constexpr int E = EW * EH;
+    constexpr int NO = (TX * TY + NT - 1) / NT;
+    constexpr int NTR = NTRC > 0 ? NTRC : 1;
+    T* const s_u = lds.u;
+    T* const s_v = lds.v;
+    T* const s_eta = lds.Pot;
+    T* const s_D = lds.D;
+    T* const s_Pot = lds.Pot;
+    T* const s_Ed = lds.Ed;
+    T* const s_pll = lds.pll;
+    T* const s_plt = lds.plt;
+    auto& s_CH = lds.CH;
+    auto& s_C = lds.C;
+    uint8_t* const s_m = lds.m;
+
+    const Statics<T>& a = *d.s;
+    const int tid = threadIdx.x;
+    const int i0 = (t % d.ntx) * TX;
+    const int j0 = (t / d.ntx) * TY;
+    const int Nx = a.Nx, Ny = a.Ny, P = a.P;
+    const bool FN = (VAR == VAR_FN);
+    // planes of this step's roles
+    T* const st = d.state;
+    const long long pl = d.plane;
+    const int r0 = d.ring, r1 = d.ring == 2 ? 0 : d.ring + 1, r2 = d.ring == 0 ? 2 : d.ring - 1;
+    const T* const su = st + (PL_U + d.cur) * pl;
+    const T* const sv = st + (PL_V + d.cur) * pl;
+    const T* const se = st + (PL_E + d.cur) * pl;
+    const T* const Gu1 = st + (PL_GU + r0) * pl;
+    const T* const Gu2 = st + (PL_GU + r1) * pl;
+    const T* const Gv1 = st + (PL_GV + r0) * pl;
+    const T* const Gv2 = st + (PL_GV + r1) * pl;
+    const T* const Ge1 = st + (PL_GE + r0) * pl;
+    const T* const Ge2 = st + (PL_GE + r1) * pl;
 
     // ---- phase 2: derived fields on the extended tile
     for (int e = tid; e < E; e += NT) {
@@ -350,6 +469,7 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
         const bool interior = DIAG && lx >= 1 && lx <= TX && ly >= 1 && ly <= TY &&
                               (i0 + lx - 1) < Nx && (j0 + ly - 1) < Ny;
         const size_t doff = interior ? (size_t)(j0 + ly - 1) * P + (i0 + lx - 1) : 0;
+        const T eta_e = s_eta[e];   // before this cell's Pot takes its place
         if (lx >= 1 && ly >= 1) {
             const int kw = e - 1, ks = e - EW, ksw = e - EW - 1;
             // Dh = interp4(D, eta2H_noland): points (i,j), (i-1,j-1), (i-1,j), (i,j-1)
@@ -384,15 +504,13 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
             s_plt[e] = plt;
             if (interior) { a.diag[1][doff] = Dh; a.diag[4][doff] = zeta; a.diag[5][doff] = Pot; }
         }
-        if (lx >= 1) {   // computeMassFluxes: MU = Du * u on ocean u
+        if (interior) {   // computeMassFluxes: MU = Du * u on ocean u, MV = Dv * v on ocean v
             const T Du = interp_noland2(s_D[e], s_D[e - 1], OE(m), OE(s_m[e - 1]));
-            s_MU[e] = OU(m) ? Du * s_u[e] : T(0);
-            if (interior) { a.diag[2][doff] = Du; a.diag[7][doff] = s_MU[e]; }
-        }
-        if (ly >= 1) {   // MV = Dv * v on ocean v
             const T Dv = interp_noland2(s_D[e], s_D[e - EW], OE(m), OE(s_m[e - EW]));
-            s_MV[e] = OV(m) ? Dv * s_v[e] : T(0);
-            if (interior) { a.diag[3][doff] = Dv; a.diag[8][doff] = s_MV[e]; }
+            a.diag[2][doff] = Du;
+            a.diag[7][doff] = OU(m) ? Du * s_u[e] : T(0);
+            a.diag[3][doff] = Dv;
+            a.diag[8][doff] = OV(m) ? Dv * s_v[e] : T(0);
         }
         if (lx <= TX && ly <= TY) {
             const int ke = e + 1, kn = e + EW;
@@ -402,9 +520,9 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
                 if (FN) {
                     const T ue = s_u[ke], uc = s_u[e], vn = s_v[kn], vc = s_v[e];
                     Ed = ((T(0.5) * (ue * ue) + T(0.5) * (uc * uc)) + (T(0.5) * (vn * vn) + T(0.5) * (vc * vc))) / T(2) +
-                         a.G * s_eta[e];
+                         a.G * eta_e;
                 } else {
-                    Ed = a.G * s_eta[e];
+                    Ed = a.G * eta_e;
                 }
             }
             s_Ed[e] = Ed;
@@ -456,6 +574,20 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
         const int oe = OE(m), ou = OU(m), ov = OV(m), oh = OH(m);
         const T Du = interp_noland2(s_D[k], s_D[kw], oe, OE(s_m[kw]));
         const T Dv = interp_noland2(s_D[k], s_D[ks], oe, OE(s_m[ks]));
+        // computeMassFluxes at the points the stencils need
+        auto MUat = [&](int kk) -> T {
+            const int mk = s_m[kk];
+            const T Du_ = interp_noland2(s_D[kk], s_D[kk - 1], OE(mk), OE(s_m[kk - 1]));
+            return OU(mk) ? Du_ * s_u[kk] : T(0);
+        };
+        auto MVat = [&](int kk) -> T {
+            const int mk = s_m[kk];
+            const T Dv_ = interp_noland2(s_D[kk], s_D[kk - EW], OE(mk), OE(s_m[kk - EW]));
+            return OV(mk) ? Dv_ * s_v[kk] : T(0);
+        };
+        const T MUk = ou ? Du * s_u[k] : T(0), MVk = ov ? Dv * s_v[k] : T(0);
+        const T MUe = MUat(ke), MUs = MUat(ks), MUse = MUat(kse);
+        const T MVn = MVat(kn), MVw = MVat(kw), MVnw = MVat(knw);
 
         // SWM_forcing_update (src/swm_forcing_module.f90:135-170)
         T Fe = fl(a.Fec, off, oe);
@@ -476,13 +608,13 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
         // SWM_timestep_nonlinear (src/swm_timestep_module.f90:349-443)
         T Ge = T(0), Gu = T(0), Gv = T(0), Lu = T(0), Lv = T(0);
         if (oe) {
-            const T t1 = dv(-(s_MU[ke] - s_MU[k]), a.dL);
-            const T t2 = dv(s_MV[kn] * rt(a.cv, jp) - s_MV[k] * rt(a.cv, j), a.dT);
+            const T t1 = dv(-(MUe - MUk), a.dL);
+            const T t2 = dv(MVn * rt(a.cv, jp) - MVk * rt(a.cv, j), a.dT);
             Ge = dv(t1 - t2, rt(a.ace, j)) + Fe;
         }
         if (ou) {
             const T P2 = T(0.5) * s_Pot[kn] + T(0.5) * s_Pot[k];
-            const T M4 = ((T(0.25) * s_MV[kn] + T(0.25) * s_MV[kw]) + T(0.25) * s_MV[knw]) + T(0.25) * s_MV[k];
+            const T M4 = ((T(0.25) * MVn + T(0.25) * MVw) + T(0.25) * MVnw) + T(0.25) * MVk;
             T g = P2 * M4;
             g = g - dv(s_Ed[k] - s_Ed[kw], rt(a.edu, j));
             if (a.quad) {
@@ -504,7 +636,7 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
         }
         if (ov) {
             const T P2 = T(0.5) * s_Pot[ke] + T(0.5) * s_Pot[k];
-            const T M4 = ((T(0.25) * s_MU[ke] + T(0.25) * s_MU[ks]) + T(0.25) * s_MU[k]) + T(0.25) * s_MU[kse];
+            const T M4 = ((T(0.25) * MUe + T(0.25) * MUs) + T(0.25) * MUk) + T(0.25) * MUse;
             T g = -(P2 * M4);
             g = g - dv(s_Ed[k] - s_Ed[ks], a.AdT);
             if (a.quad) {
@@ -532,8 +664,8 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
             for (int n = 0; n < NTRC; ++n) {
                 const T C = s_C[n][k];
                 a.tdiag[n][0][off] = C;
-                a.tdiag[n][1][off] = s_MU[k] * T(0.5) * (C + s_C[n][kw]);
-                a.tdiag[n][2][off] = s_MV[k] * T(0.5) * (C + s_C[n][ks]);
+                a.tdiag[n][1][off] = MUk * T(0.5) * (C + s_C[n][kw]);
+                a.tdiag[n][2][off] = MVk * T(0.5) * (C + s_C[n][ks]);
             }
             continue;
         }
@@ -543,7 +675,7 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
         const T Gu3 = ou ? Gu : pg[q][1];
         const T Gv3 = ov ? Gv : pg[q][3];
         const T Ge3 = oe ? Ge : pg[q][5];
-        T un = s_u[k], vn = s_v[k], en = s_eta[k];
+        T un = s_u[k], vn = s_v[k], en = pe[q];
         un = madd(un, d.dtc1, pg[q][0]);
         vn = madd(vn, d.dtc1, pg[q][2]);
         en = madd(en, d.dtc1, pg[q][4]);
@@ -623,11 +755,57 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
     }
 }
 
+// One 256-thread block advances tile t (TX x TY cells) of the domain in d by
+// one step.  L is a TileLds with room for at least NTRC tracers.
+template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, typename L>
+__device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) {
+    using S = TileShapeC<TX, TY>;
+    TileRegs<T, S::NE, S::NO, (NTRC > 0 ? NTRC : 1)> R;
+    tile_load<T, VAR, NTRC, TX, TY, DIAG>(d, t, R);
+    tile_stage<T, VAR, NTRC, TX, TY, DIAG>(d, t, R, lds);
+    __syncthreads();
+    tile_compute<T, VAR, NTRC, TX, TY, DIAG>(d, t, lds, R.g, R.gt, R.eo);
+}
+
 template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG>
 __global__ void __launch_bounds__(256)
 step_kernel(const Dyn<T> d) {
     __shared__ TileLds<T, (NTRC > 0 ? NTRC : 1), (TX + 2) * (TY + 2), TY + 2> lds;
     tile_body<T, VAR, NTRC, TX, TY, DIAG>(d, xcd_tile(blockIdx.x, d.nblk), lds);
+}
+
+// Persistent tile kernel: block b advances the tiles [b*tpb, (b+1)*tpb) in
+// order (neighbours along x, so halo columns are shared in L2) and issues the
+// global loads of tile k+1 before it computes tile k, so the memory latency of
+// one tile overlaps the arithmetic of the previous one.
+template <typename T, int VAR, int NTRC, int TX, int TY>
+__global__ void __launch_bounds__(256)
+step_kernel_p(const Dyn<T> d) {
+    using S = TileShapeC<TX, TY>;
+    constexpr int NTR = NTRC > 0 ? NTRC : 1;
+    __shared__ TileLds<T, NTR, S::E, S::EH> lds;
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    const int t0 = b * d.tpb;
+    const int t1 = t0 + d.tpb < d.nblk ? t0 + d.tpb : d.nblk;
+    if (t0 >= t1) return;   // whole block
+    TileRegs<T, S::NE, S::NO, NTR> R;
+    tile_load<T, VAR, NTRC, TX, TY, false>(d, t0, R);
+    for (int t = t0; t < t1; ++t) {
+        if (t > t0) __syncthreads();   // phase 3 of the previous tile is done with LDS
+        tile_stage<T, VAR, NTRC, TX, TY, false>(d, t, R, lds);
+        T g[S::NO][6], gt[S::NO][NTR][2], eo[S::NO];
+#pragma unroll
+        for (int q = 0; q < S::NO; ++q) {
+            eo[q] = R.eo[q];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) g[q][k] = R.g[q][k];
+#pragma unroll
+            for (int n = 0; n < NTR; ++n) { gt[q][n][0] = R.gt[q][n][0]; gt[q][n][1] = R.gt[q][n][1]; }
+        }
+        __syncthreads();
+        if (t + 1 < t1) tile_load<T, VAR, NTRC, TX, TY, false>(d, t + 1, R);
+        tile_compute<T, VAR, NTRC, TX, TY, false>(d, t, lds, g, gt, eo);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -656,7 +834,7 @@ struct MemberStep {
 };
 
 template <typename T, int VAR, int NTRC_MAX, int TX, int TY>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 batch_kernel(const Member<T>* __restrict__ mem, const MemberStep* __restrict__ stp, const int* __restrict__ owner,
              int nwork) {
     __shared__ TileLds<T, (NTRC_MAX > 0 ? NTRC_MAX : 1), (TX + 2) * (TY + 2), TY + 2> lds;
@@ -1170,11 +1348,12 @@ sor_check(const double* partial, int nblk, int sweep, double eps, double bnorm, 
     }
 }
 
-// computeDivergence(u, v, div) (src/calc_lib.f90:607-622)
-template <typename T>
+// computeDivergence(u, v, div) (src/calc_lib.f90:607-622), evaluated in the
+// field precision T and stored in TO (the SOR right-hand side precision).
+template <typename T, typename TO = T>
 __global__ void __launch_bounds__(256)
 divergence_kernel(int Nx, int Ny, int P, const T* u, const T* v, const uint8_t* mask, const T* dza,
-                  const T* cv, const T* ce, T AdT, T lbc, T* div) {
+                  const T* cv, const T* ce, T AdT, T lbc, TO* div) {
     const long idx = (long)blockIdx.x * 256 + threadIdx.x;
     if (idx >= (long)Nx * Ny) return;
     const int i = (int)(idx % Nx), j = (int)(idx / Nx);
@@ -1188,7 +1367,46 @@ divergence_kernel(int Nx, int Ny, int P, const T* u, const T* v, const uint8_t* 
         const T bc = (m & OC_E) ? T(1) : lbc;
         zb = bc * (cv[jp] * v[(size_t)jp * P + i] - cv[j] * v[off]) / AdT;
     }
-    div[off] = za + zb / ce[j];
+    div[off] = (TO)(za + zb / ce[j]);
+}
+
+// sum over ocean eta points of B^2 in FP64, fixed order: per-block partials
+// (one 256-thread tree per block) then one block over the partials.
+template <typename TS>
+__global__ void __launch_bounds__(256)
+sq_norm_kernel(int Nx, int Ny, int P, const TS* B, const uint8_t* mask, double* partial) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    const long n = (long)Nx * Ny;
+    for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (long)gridDim.x * 256) {
+        const int j = (int)(idx / Nx), i = (int)(idx - (long)j * Nx);
+        const size_t off = (size_t)j * P + i;
+        if (mask[off] & OC_E) {
+            const double b = (double)B[off];
+            acc += b * b;
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256)
+sq_norm_final(const double* partial, int nblk, double* out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256) acc += partial[b];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sqrt(red[0]);
 }
 
 // ---------------------------------------------------------------------------

@@ -29,7 +29,8 @@ FP64, FP32 = 0, 1
 
 EXPORTED = [
     "swm_gpu_create", "swm_gpu_upload_state", "swm_gpu_add_tracer", "swm_gpu_step", "swm_gpu_itt",
-    "swm_gpu_sync", "swm_gpu_sor", "swm_gpu_sor_fixed", "swm_gpu_divergence", "swm_gpu_sums",
+    "swm_gpu_sync", "swm_gpu_sor", "swm_gpu_sor_fixed", "swm_gpu_velocity_potential", "swm_gpu_divergence",
+    "swm_gpu_sums",
     "swm_gpu_last_timing", "swm_gpu_last_error", "swm_gpu_destroy",
     "swm_gpu_batch_create", "swm_gpu_batch_attach", "swm_gpu_batch_detach", "swm_gpu_batch_set_steps",
     "swm_gpu_batch_remaining", "swm_gpu_batch_step", "swm_gpu_batch_last_timing", "swm_gpu_batch_last_error",
@@ -96,6 +97,9 @@ def load_library(path: str = LIB_PATH):
     L.swm_gpu_sor_fixed.argtypes = [p, _dp, _dp, C.c_double, C.c_double, C.c_int64, C.c_int32,
                                     C.POINTER(C.c_int64), _dp]
     L.swm_gpu_sor_fixed.restype = C.c_int
+    L.swm_gpu_velocity_potential.argtypes = [p, C.c_double, C.c_double, C.c_int64, C.c_int32, C.c_int32,
+                                             C.POINTER(C.c_int64), _dp]
+    L.swm_gpu_velocity_potential.restype = C.c_int
     L.swm_gpu_divergence.argtypes = [p, _dp]
     L.swm_gpu_divergence.restype = C.c_int
     L.swm_gpu_sums.argtypes = [p, C.c_int32, _dp]
@@ -287,6 +291,16 @@ class SwmGpu:
         out = np.empty((self.s.Ny, self.s.Nx), dtype=np.float64)
         self._check(self.L.swm_gpu_divergence(self._h, out.ctypes.data_as(_dp)), "swm_gpu_divergence")
         return out
+
+    def velocity_potential(self, omega=1.9, rel_tol=1e-10, max_sweeps=5000, fp32_sweeps=False, reset=False):
+        """Device-resident computeVelocityPotential: returns (iters, rel_res, converged);
+        the solution stays on the device (get("CHI"))."""
+        it, rr = C.c_int64(), C.c_double()
+        rc = self.L.swm_gpu_velocity_potential(self._h, float(omega), float(rel_tol), int(max_sweeps),
+                                               int(bool(fp32_sweeps)), int(bool(reset)), C.byref(it), C.byref(rr))
+        if rc not in (0, 1):
+            self._check(rc, "swm_gpu_velocity_potential")
+        return it.value, rr.value, rc == 0
 
     def sums(self, tracer: int = -1):
         """FP64 compensated (volume, energy, tracer inventory) sums, cos-weighted

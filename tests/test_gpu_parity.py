@@ -228,9 +228,9 @@ def test_sor_fixed_omega_fp64_and_fp32_sweeps(gpu_lib):
 
 def test_every_kernel_shape_bit_identical(gpu_lib, monkeypatch):
     """Every selectable step-kernel shape (LDS tiles 64x4, 32x8, 64x8, 128x4,
-    32x16 and the marching-wavefront kernel with automatic and fixed row
-    counts) gives the oracle's result bit for bit, on a grid that no tile
-    divides, with two tracers."""
+    32x16, the marching-wavefront kernel with automatic and fixed row counts,
+    and the persistent 64x4 / 32x8 tile kernels) gives the oracle's result bit
+    for bit, on a grid that no tile divides, with two tracers."""
     s = configs.basin(141, 99, 5e3, 5e3, variant=host.FULLY_NONLINEAR, H0=400.0, dt=8.0, Ah=120.0, f0=1e-4,
                       beta=2e-11, drag=1e-6, lateral_mixing=True, land_fraction=0.06, seed=31)
     host.add_windstress(s, host.COMP_X, configs.double_gyre_tau(s, 0.1))
@@ -239,9 +239,43 @@ def test_every_kernel_shape_bit_identical(gpu_lib, monkeypatch):
     host.finalize(s)
     o = oracle.Oracle(s)
     o.step(9)
-    for tile in range(9):
+    for tile in range(11):
         monkeypatch.setenv("SWM_GPU_TILE", str(tile))
         g = gpu.SwmGpu(s).init_from_setup()
         g.step(9)
         compare(g, o, STATE, 2, f"tile {tile}")
         g.close()
+
+
+def test_velocity_potential_device_resident(gpu_lib):
+    """computeVelocityPotential on the device (divergence -> SOR, no host
+    copies) matches the oracle's fixed-omega SOR on the oracle's divergence;
+    a warm start from the previous step's solution needs fewer sweeps; FP32
+    sweeps with the FP64 norm stay close to the FP64 solution."""
+    s = configs.c2_nonlinear(seed=2, N=96, dt=10.0, tracer=False)
+    g = gpu.SwmGpu(s).init_from_setup()
+    o = oracle.Oracle(s)
+    g.step(15)
+    o.step(15)
+    B = o.divergence(o.get("u"), o.get("v"))
+    chi_o, it_o, rr_o, rc_o = o.sor_fixed(B, np.zeros(s.shape), 1.9, 1e-10, 5000)
+    it_g, rr_g, ok = g.velocity_potential(1.9, 1e-10, 5000, reset=True)
+    assert ok and rc_o == 0
+    assert abs(it_g - it_o) <= 1
+    if it_g == it_o:
+        assert same(g.get("CHI"), chi_o)
+    chi64 = g.get("CHI")
+    # FP32 sweeps: chi in FP32 bounds the attainable relative residual (about
+    # 1e-6 here), so the run is judged by its residual and its distance to FP64
+    it_f, rr_f, ok_f = g.velocity_potential(1.9, 1e-6, 5000, fp32_sweeps=True, reset=True)
+    assert rr_f < 1e-4
+    oc = s.grids["eta"].ocean == 1
+    ref = chi64[oc] - chi64[oc].mean()
+    got = g.get("CHI")[oc]
+    got = got - got.mean()
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-3
+    g.step(1)
+    it_w, _, ok_w = g.velocity_potential(1.9, 1e-10, 5000)
+    it_z, _, ok_z = g.velocity_potential(1.9, 1e-10, 5000, reset=True)
+    assert ok_w and ok_z and it_w < it_z
+    g.close()
