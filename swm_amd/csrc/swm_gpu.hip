@@ -1259,6 +1259,7 @@ struct swm_batch {
     size_t owner_cap = 0;
     int nwork = 0, ntrc_max = 0;
     hipGraphExec_t graph = nullptr;
+    bool use_graph = true;               // SWM_GPU_GRAPH=0: launch each step directly (profiling)
     double last_ms = 0.0;
     std::string err;
 };
@@ -1400,6 +1401,7 @@ int swm_gpu_batch_create(swm_batch** out) {
     if (!out) return 1;
     *out = nullptr;
     swm_batch* b = new swm_batch();
+    if (const char* e = getenv("SWM_GPU_GRAPH")) b->use_graph = atoi(e) != 0;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
         g_create_err = "batch: could not create a HIP stream/events";
@@ -1483,7 +1485,17 @@ int swm_gpu_batch_step(swm_batch* b, int64_t nsteps, int32_t* active) {
         if (rc) return rc;
     }
     b->last_ms = 0.0;
-    if (!b->mem.empty() && nsteps > 0) {
+    if (!b->mem.empty() && nsteps > 0 && !b->use_graph) {
+        BHIPCHK(b, hipEventRecord(b->ev0, b->stream));
+        for (int64_t k = 0; k < nsteps; ++k) batch_launch(b);
+        BHIPCHK(b, hipGetLastError());
+        BHIPCHK(b, hipEventRecord(b->ev1, b->stream));
+        BHIPCHK(b, hipEventSynchronize(b->ev1));
+        float ms = 0.f;
+        BHIPCHK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->last_ms = ms;
+        b->on_device = true;
+    } else if (!b->mem.empty() && nsteps > 0) {
         if (!b->graph) {
             HipLock lock(g_hip_mu);
             hipGraph_t g = nullptr;

@@ -189,6 +189,21 @@ struct TileLds {
     uint8_t m[E];
 };
 
+// Global loads and stores through an address-space-1 pointer, and the statics
+// through the constant address space.  Pointers that reach a kernel through
+// memory (the batched kernel reads them from its member table) are otherwise
+// generic and every access would become a flat instruction.
+template <typename T>
+__device__ __forceinline__ T gld(const T* p, size_t i) {
+    return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, size_t i, T v) {
+    ((__attribute__((address_space(1))) T*)p)[i] = v;
+}
+template <typename T>
+using CStatics = const __attribute__((address_space(4))) Statics<T>;
+
 // Registers of one tile between its loads and their LDS stores: the state with
 // halo (NE cells per thread), the per-row tables and the AB history of the
 // thread's NO output cells.
@@ -218,7 +233,7 @@ __device__ __forceinline__ void tile_load(const Dyn<T>& d, const int t, RG& R) {
     constexpr int E = EW * EH;
     constexpr int NO = (TX * TY + NT - 1) / NT;
     constexpr int NTR = NTRC > 0 ? NTRC : 1;
-    const Statics<T>& a = *d.s;
+    CStatics<T>& a = *(CStatics<T>*)d.s;
     const int tid = threadIdx.x;
     const int i0 = (t % d.ntx) * TX;
     const int j0 = (t / d.ntx) * TY;
@@ -248,15 +263,15 @@ __device__ __forceinline__ void tile_load(const Dyn<T>& d, const int t, RG& R) {
             const int i = i0 + o % TX, j = j0 + o / TX;
             if (o < TX * TY && i < Nx && j < Ny) {
                 const size_t off = (size_t)j * P + i;
-                pg[q][0] = Gu1[off]; pg[q][1] = Gu2[off];
-                pg[q][2] = Gv1[off]; pg[q][3] = Gv2[off];
-                pg[q][4] = Ge1[off]; pg[q][5] = Ge2[off];
-                R.eo[q] = se[off];
+                pg[q][0] = gld(Gu1, off); pg[q][1] = gld(Gu2, off);
+                pg[q][2] = gld(Gv1, off); pg[q][3] = gld(Gv2, off);
+                pg[q][4] = gld(Ge1, off); pg[q][5] = gld(Ge2, off);
+                R.eo[q] = gld(se, off);
 #pragma unroll
                 for (int n = 0; n < NTRC; ++n) {
                     const T* tb = st + (PL_TRC + PL_PER_TRC * n + 2) * pl;
-                    pt[q][n][0] = tb[r0 * pl + off];
-                    pt[q][n][1] = tb[r1 * pl + off];
+                    pt[q][n][0] = gld(tb, r0 * pl + off);
+                    pt[q][n][1] = gld(tb, r1 * pl + off);
                 }
             }
         }
@@ -281,18 +296,18 @@ __device__ __forceinline__ void tile_load(const Dyn<T>& d, const int t, RG& R) {
             const int lx = e % EW, ly = e / EW;
             const int gi = wrap(i0 + lx - 1, Nx), gj = wrap(j0 + ly - 1, Ny);
             const size_t off = (size_t)gj * P + gi;
-            rm[q] = a.mask[off];
-            ru[q] = su[off];
-            rv[q] = sv[off];
-            re[q] = se[off];
+            rm[q] = gld(a.mask, off);
+            ru[q] = gld(su, off);
+            rv[q] = gld(sv, off);
+            re[q] = gld(se, off);
             // H_eta at an ocean (rHo) and a land (rHl) point of this cell's row
-            rHo[q] = a.He.full ? a.He.full[off] : a.He.rows[2 * gj];
-            rHl[q] = a.He.full ? rHo[q] : a.He.rows[2 * gj + 1];
+            rHo[q] = a.He.full ? gld(a.He.full, off) : gld(a.He.rows, 2 * (size_t)gj);
+            rHl[q] = a.He.full ? rHo[q] : gld(a.He.rows, 2 * (size_t)gj + 1);
 #pragma unroll
             for (int n = 0; n < NTRC; ++n) {
                 const T* tb = st + (PL_TRC + PL_PER_TRC * n) * pl;
-                rch[n][q] = tb[d.cur * pl + off];
-                rcb[n][q] = ((d.cbuf >> n) & 1) ? tb[5 * pl + off] : T(0);
+                rch[n][q] = gld(tb, d.cur * pl + off);
+                rcb[n][q] = ((d.cbuf >> n) & 1) ? gld(tb, 5 * pl + off) : T(0);
             }
         }
     }
@@ -306,18 +321,18 @@ __device__ __forceinline__ void tile_load(const Dyn<T>& d, const int t, RG& R) {
     T& tpc = R.tpc;
     if (tid < EH) {
         const int gj = wrap(j0 + tid - 1, Ny);
-        tz0 = a.za[2 * gj];
-        tz1 = a.za[2 * gj + 1];
-        tcu = a.cu[gj];
-        tcH = a.cH[gj];
-        tfH = a.fH[gj];
+        tz0 = gld(a.za, 2 * gj);
+        tz1 = gld(a.za, 2 * gj + 1);
+        tcu = gld(a.cu, gj);
+        tcH = gld(a.cH, gj);
+        tfH = gld(a.fH, gj);
     }
-    if (tid == NT - 1) tcu0 = a.cu[wrap(j0 - 2, Ny)];
+    if (tid == NT - 1) tcu0 = gld(a.cu, wrap(j0 - 2, Ny));
     const bool has_pc = a.mix && tid < EH * 24;
     if (has_pc) {
         const int r = tid / 24, c = tid % 24;
         const int gj = wrap(j0 + r - 1, Ny);
-        tpc = c < 12 ? a.pltc[gj * 12 + c] : a.pllc[gj * 12 + c - 12];
+        tpc = c < 12 ? gld(a.pltc, gj * 12 + c) : gld(a.pllc, gj * 12 + c - 12);
     }
 }
 
@@ -342,7 +357,7 @@ __device__ __forceinline__ void tile_stage(const Dyn<T>& d, const int t, const R
     auto& s_C = lds.C;
     uint8_t* const s_m = lds.m;
 
-    const Statics<T>& a = *d.s;
+    CStatics<T>& a = *(CStatics<T>*)d.s;
     const int tid = threadIdx.x;
     const int i0 = (t % d.ntx) * TX;
     const int j0 = (t / d.ntx) * TY;
@@ -413,8 +428,8 @@ __device__ __forceinline__ void tile_stage(const Dyn<T>& d, const int t, const R
         for (int q = tid + NT; q < EH * 24; q += NT) {
             const int r = q / 24, c = q % 24;
             const int gj = wrap(j0 + r - 1, Ny);
-            if (c < 12) lds.pltc[r][c] = a.pltc[gj * 12 + c];
-            else lds.pllc[r][c - 12] = a.pllc[gj * 12 + c - 12];
+            if (c < 12) lds.pltc[r][c] = gld(a.pltc, gj * 12 + c);
+            else lds.pllc[r][c - 12] = gld(a.pllc, gj * 12 + c - 12);
         }
 }
 
@@ -441,7 +456,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
     auto& s_C = lds.C;
     uint8_t* const s_m = lds.m;
 
-    const Statics<T>& a = *d.s;
+    CStatics<T>& a = *(CStatics<T>*)d.s;
     const int tid = threadIdx.x;
     const int i0 = (t % d.ntx) * TX;
     const int j0 = (t / d.ntx) * TY;
@@ -502,15 +517,15 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                 plt = Dh * ((c[0] * (s_v[e] - s_v[kw]) + c[1] * s_u[e]) + c[2] * s_u[ks]);
             }
             s_plt[e] = plt;
-            if (interior) { a.diag[1][doff] = Dh; a.diag[4][doff] = zeta; a.diag[5][doff] = Pot; }
+            if (interior) { gst(a.diag[1], doff, Dh); gst(a.diag[4], doff, zeta); gst(a.diag[5], doff, Pot); }
         }
         if (interior) {   // computeMassFluxes: MU = Du * u on ocean u, MV = Dv * v on ocean v
             const T Du = interp_noland2(s_D[e], s_D[e - 1], OE(m), OE(s_m[e - 1]));
             const T Dv = interp_noland2(s_D[e], s_D[e - EW], OE(m), OE(s_m[e - EW]));
-            a.diag[2][doff] = Du;
-            a.diag[7][doff] = OU(m) ? Du * s_u[e] : T(0);
-            a.diag[3][doff] = Dv;
-            a.diag[8][doff] = OV(m) ? Dv * s_v[e] : T(0);
+            gst(a.diag[2], doff, Du);
+            gst(a.diag[7], doff, OU(m) ? Du * s_u[e] : T(0));
+            gst(a.diag[3], doff, Dv);
+            gst(a.diag[8], doff, OV(m) ? Dv * s_v[e] : T(0));
         }
         if (lx <= TX && ly <= TY) {
             const int ke = e + 1, kn = e + EW;
@@ -533,7 +548,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                 pll = s_D[e] * ((c[0] * (s_u[ke] - s_u[e]) + c[1] * s_v[kn]) + c[2] * s_v[e]);
             }
             s_pll[e] = pll;
-            if (interior) { a.diag[6][doff] = Ed; a.diag[0][doff] = s_D[e]; }
+            if (interior) { gst(a.diag[6], doff, Ed); gst(a.diag[0], doff, s_D[e]); }
         }
     }
     __syncthreads();
@@ -551,15 +566,15 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
         if (o >= TX * TY || i >= Nx || j >= Ny) continue;
         auto rt = [&](const T* p, int idx) -> T {   // per-row table value
             if constexpr (UROW) return ldc(p, (size_t)idx);
-            else return p[idx];
+            else return gld(p, (size_t)idx);
         };
-        auto fl = [&](const Fld<T>& f, size_t at, bool ocean) -> T {   // static field at (i, j)
+        auto fl = [&](auto& f, size_t at, bool ocean) -> T {   // static field at (i, j)
             if constexpr (UROW) {
-                if (f.full) return f.full[at];
+                if (f.full) return gld(f.full, at);
                 const T fo = ldc(f.rows, 2 * (size_t)j), fl_ = ldc(f.rows, 2 * (size_t)j + 1);
                 return ocean ? fo : fl_;
             } else {
-                return fld(f, at, j, ocean);
+                return f.full ? gld(f.full, at) : gld(f.rows, 2 * (size_t)j + (ocean ? 0 : 1));
             }
         };
         auto ta = [&](bool neg, int nummask, int den, const T* tab) -> T {
@@ -658,14 +673,14 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
         }
 
         if (DIAG) {
-            a.diag[9][off] = Fx; a.diag[10][off] = Fy; a.diag[11][off] = Fe;
-            a.diag[12][off] = Lu; a.diag[13][off] = Lv;
+            gst(a.diag[9], off, Fx); gst(a.diag[10], off, Fy); gst(a.diag[11], off, Fe);
+            gst(a.diag[12], off, Lu); gst(a.diag[13], off, Lv);
 #pragma unroll
             for (int n = 0; n < NTRC; ++n) {
                 const T C = s_C[n][k];
-                a.tdiag[n][0][off] = C;
-                a.tdiag[n][1][off] = MUk * T(0.5) * (C + s_C[n][kw]);
-                a.tdiag[n][2][off] = MVk * T(0.5) * (C + s_C[n][ks]);
+                gst(a.tdiag[n][0], off, C);
+                gst(a.tdiag[n][1], off, MUk * T(0.5) * (C + s_C[n][kw]));
+                gst(a.tdiag[n][2], off, MVk * T(0.5) * (C + s_C[n][ks]));
             }
             continue;
         }
@@ -696,12 +711,12 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             const T lim = a.minD - fl(a.He, off, oe);
             if (en < lim) en = lim;
         }
-        st[(PL_U + 1 - d.cur) * pl + off] = un;
-        st[(PL_V + 1 - d.cur) * pl + off] = vn;
-        st[(PL_E + 1 - d.cur) * pl + off] = en;
-        st[(PL_GU + r2) * pl + off] = Gu3;
-        st[(PL_GV + r2) * pl + off] = Gv3;
-        st[(PL_GE + r2) * pl + off] = Ge3;
+        gst(st, (PL_U + 1 - d.cur) * pl + off, un);
+        gst(st, (PL_V + 1 - d.cur) * pl + off, vn);
+        gst(st, (PL_E + 1 - d.cur) * pl + off, en);
+        gst(st, (PL_GU + r2) * pl + off, Gu3);
+        gst(st, (PL_GV + r2) * pl + off, Gv3);
+        gst(st, (PL_GE + r2) * pl + off, Ge3);
 
         // TRC_tracer_incr + TRC_tracer_integrate (src/tracer_module.f90:154-232)
 #pragma unroll
@@ -749,8 +764,8 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             if (d.tstep >= 2) chn = madd(chn, d.dtc2, pt[q][n][1]);
             if (d.tstep >= 3) chn = madd(chn, d.dtc3, g);
             chn = dv(chn, fl(a.timpl[n], off, oe));
-            tb[(1 - d.cur) * pl + off] = chn;
-            tb[(2 + r2) * pl + off] = g;
+            gst(tb, (1 - d.cur) * pl + off, chn);
+            gst(tb, (2 + r2) * pl + off, g);
         }
     }
 }
