@@ -106,18 +106,20 @@ def plain_fp64_rate(steps: int = 1000) -> float:
 
 
 def run_ensemble(setups, precision, warmup, steps):
-    """Create, attach and advance every member; returns (members, device ms of the
-    timed steps)."""
-    from swm_amd import gpu
+    """Create and attach every member, let the autotuner pick the batch tile
+    shape (from a snapshot, state restored), advance; returns (members, device
+    ms of the timed steps, tuning result)."""
+    from swm_amd import autotune, gpu
     ms = [gpu.SwmGpu(s, precision).init_from_setup() for s in setups]
     b = gpu.SwmBatch()
     for m in ms:
         b.attach(m, warmup + steps)
+    tuned = autotune.tune_batch(b, steps=4)
     b.step(warmup)
     b.step(steps)
     t = b.last_ms()
     b.close()
-    return ms, t
+    return ms, t, tuned
 
 
 def bench_ensemble(args):
@@ -131,8 +133,8 @@ def bench_ensemble(args):
     ntr = [len(s.tracers) for s in setups]
     bytes_f64 = sum(c * (121 + 40 * t) for c, t in zip(cells, ntr))
     bytes_f32 = sum(c * (61 + 20 * t) for c, t in zip(cells, ntr))
-    m64, ms64 = run_ensemble(setups, gpu.FP64, args.warmup, args.steps)
-    m32, ms32 = run_ensemble(setups, gpu.FP32, args.warmup, args.steps)
+    m64, ms64, tune64 = run_ensemble(setups, gpu.FP64, args.warmup, args.steps)
+    m32, ms32, tune32 = run_ensemble(setups, gpu.FP32, args.warmup, args.steps)
     err, drift = [], []
     for s, a, b in zip(setups, m64, m32):
         oc = s.grids["eta"].ocean == 1
@@ -159,6 +161,7 @@ def bench_ensemble(args):
                      "traffic": None, "bytes_per_step": bytes_f32},
         "fp64": {"value": v64, "ms_per_step": ms64 / args.steps,
                  "achieved_GBs": bytes_f64 / (ms64 * 1e-3 / args.steps) / 1e9},
+        "autotune": {"f32": tune32, "f64": tune64, "shapes": ["64x4", "32x8"]},
         "plain_fp64_c2_value": plain,
         "fp32_vs_plain_fp64": v32 / plain,
         "fp32_error_vs_fp64": {"eta_rel_maxnorm_max": max(err), "eta_rel_maxnorm_median": float(np.median(err)),
@@ -226,9 +229,11 @@ def bench_elliptic(args):
     the FP64 norm, each with its own warm-start chain on the same trajectory."""
     import numpy as np
     from swm_amd import configs, gpu
+    from swm_amd import autotune
     s = configs.c3_elliptic(seed=0, nsteps=args.steps)
     g = gpu.SwmGpu(s).init_from_setup()
     g.step(args.warmup)
+    tuned = autotune.tune_sor(g, omega=1.9, sweeps=128)
     oc = s.grids["eta"].ocean == 1
     rec = {"f64": [], "f32": []}
     t_sor = {"f64": 0.0, "f32": 0.0}
@@ -265,6 +270,7 @@ def bench_elliptic(args):
         "sweep_us": {t: per_sweep[t] * 1e6 for t in per_sweep},
         "sweep_GBs_algorithmic": {t: bytes_sweep[t] / per_sweep[t] / 1e9 for t in per_sweep},
         "fp32_sweeps_chi_rel_error_vs_fp64": err,
+        "autotune_sor": tuned,
         "model_step_ms": t_step / args.steps * 1e3,
     }
     print(json.dumps(out))

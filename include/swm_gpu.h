@@ -146,6 +146,22 @@ int swm_gpu_sor(swm_ctx *c, const double *B, double *chi, double epsilon, int32_
 int swm_gpu_sor_fixed(swm_ctx *c, const double *B, double *chi, double omega, double rel_tol,
                       int64_t max_sweeps, int32_t fp32_sweeps, int64_t *iters, double *rel_res);
 
+/* Tuning knobs and the autotuner.  No counterpart in the reference (its loops
+ * have no launch shapes); they serve callers that pick kernel shapes for their
+ * grids.  Shapes 0..swm_gpu_kernel_shape_count()-1: 0 = 64x4 LDS tiles
+ * (default), 1 = 32x8, 2 = 64x8, 3 = 128x4, 4 = 32x16, 5..8 = marching
+ * wavefronts (automatic, 8, 16, 4 rows), 9, 10 = persistent 64x4, 32x8 tiles;
+ * every shape gives the same FP64 results bit for bit.  swm_gpu_tune times
+ * `steps` steps of every shape from a device snapshot of the state, restores
+ * the state exactly, selects the fastest and returns it (times_ms, if not
+ * NULL, receives kernel_shape_count per-step times).  set_sor_check_every
+ * makes the fixed-omega SOR test convergence every `every` sweeps (1..64;
+ * the reference mode always tests every sweep). */
+int swm_gpu_kernel_shape_count(void);
+int swm_gpu_set_kernel_shape(swm_ctx *c, int32_t shape);
+int swm_gpu_tune(swm_ctx *c, int32_t steps, int32_t *best_shape, double *times_ms);
+int swm_gpu_set_sor_check_every(swm_ctx *c, int32_t every);
+
 /* computeVelocityPotential (src/calc_lib.f90:500-525) without host copies:
  * B = computeDivergence of the current device velocity (calc_lib.f90:607-622)
  * in the sweep precision, then fixed-omega red-black SOR as in
@@ -198,6 +214,11 @@ int swm_gpu_batch_detach(swm_batch *b, swm_ctx *c);
 int swm_gpu_batch_set_steps(swm_batch *b, swm_ctx *c, int64_t nsteps);
 int swm_gpu_batch_remaining(swm_batch *b, const swm_ctx *c, int64_t *remaining);
 int swm_gpu_batch_step(swm_batch *b, int64_t nsteps, int32_t *active);
+/* batch tile shape: 0 = 64x4 (default), 1 = 32x8; batch_tune times `steps`
+ * steps of each from a device snapshot of every member, restores the members
+ * exactly, selects the fastest (times_ms: 2 per-step times, may be NULL). */
+int swm_gpu_batch_set_shape(swm_batch *b, int32_t shape);
+int swm_gpu_batch_tune(swm_batch *b, int32_t steps, int32_t *best_shape, double *times_ms);
 /* device time of the last swm_gpu_batch_step (ms) */
 int swm_gpu_batch_last_timing(const swm_batch *b, double *total_ms);
 const char *swm_gpu_batch_last_error(const swm_batch *b);

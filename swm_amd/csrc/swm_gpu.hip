@@ -101,6 +101,7 @@ struct swm_ctx {
     double* sums_buf = nullptr;  // conservation-sum partials and result
     double* norm_buf = nullptr;  // ||B|| partials and result of the device-resident SOR
     bool chi_valid[2] = {};      // sor_buf[w][0] holds a solution to warm-start from
+    int sor_every = 1;           // fixed-omega mode: convergence test every sor_every sweeps
     int chi_last = -1;           // precision slot of the last solution
     double* ce64 = nullptr;      // cos(lat_eta) rows in FP64
     std::vector<double> ce_host;
@@ -995,12 +996,14 @@ int sor_prepare(swm_ctx* c, bool f32, int nblk) {
 template <typename T>
 int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mode, double omega, double tol,
                       double bnorm, int64_t max_sweeps, int nblk, int* host_done) {
-    const int batch = 32;
+    const int every = ref_mode ? 1 : c->sor_every;   // the reference tests after every sweep
+    const int batch = std::max(32, 4 * every);
     *host_done = 0;
     int64_t l = 1;
     while (l <= max_sweeps && !*host_done) {
         const int64_t lend = std::min<int64_t>(max_sweeps, l + batch - 1);
         for (; l <= lend; ++l) {
+            const bool check = (l % every) == 0 || l == max_sweeps;
             for (int color = 0; color < 2; ++color) {
                 swm::SorArgs<T> a{};
                 a.Nx = c->Nx; a.Ny = c->Ny; a.P = c->P; a.color = color; a.sweep = (int)l;
@@ -1011,10 +1014,13 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
                 a.partial = c->sor_partial + color * nblk;
                 a.done = c->sor_done;
                 a.residual_mode = ref_mode ? 0 : 1;
+                a.accumulate = check ? 1 : 0;
                 hipLaunchKernelGGL(swm::sor_sweep<T>, dim3(nblk), dim3(256), 0, c->stream, a);
             }
-            hipLaunchKernelGGL(swm::sor_check, dim3(1), dim3(256), 0, c->stream, (const double*)c->sor_partial, 2 * nblk,
-                               (int)l, tol, bnorm, ref_mode ? 0 : 1, c->sor_done, c->sor_iters, c->sor_norm);
+            if (check)
+                hipLaunchKernelGGL(swm::sor_check, dim3(1), dim3(256), 0, c->stream, (const double*)c->sor_partial,
+                                   2 * nblk, (int)l, tol, bnorm, ref_mode ? 0 : 1, c->sor_done, c->sor_iters,
+                                   c->sor_norm);
         }
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipMemcpyAsync(host_done, c->sor_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1148,6 +1154,81 @@ int swm_gpu_sor_fixed(swm_ctx* c, const double* B, double* chi, double omega, do
     return sor_common(c, B, chi, 0, omega, rel_tol, max_sweeps, fp32_sweeps, iters, rel_res);
 }
 
+int swm_gpu_set_kernel_shape(swm_ctx* c, int32_t shape) {
+    if (!c) return 1;
+    if (shape < 0 || shape >= kNumTiles) return fail(c, "no such kernel shape", 1);
+    HipLock lock(g_hip_mu);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    c->tile = shape;
+    return 0;
+}
+
+int swm_gpu_kernel_shape_count(void) { return kNumTiles; }
+
+int swm_gpu_tune(swm_ctx* c, int32_t steps, int32_t* best_shape, double* times_ms) {
+    if (!c || !best_shape) return 1;
+    if (c->batch) return fail(c, "context is attached to a batch; tune the batch instead", 1);
+    if (steps < 1) return fail(c, "steps must be >= 1", 1);
+    HipLock lock(g_hip_mu);
+    const int ntrc = (int)c->trc.size();
+    const size_t bytes = (size_t)(swm::PL_TRC + swm::PL_PER_TRC * ntrc) * c->plane * c->esz;
+    void* snap = nullptr;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (hipMalloc(&snap, bytes) != hipSuccess) return fail(c, "out of device memory for the tuning snapshot", 3);
+    const int64_t itt0 = c->itt;
+    const int cur0 = c->cur, ring0 = c->ring;
+    std::vector<bool> cb0;
+    for (auto& t : c->trc) cb0.push_back(t.cbuf_valid);
+    const int tile0 = c->tile;
+    const bool graph0 = c->use_graph;
+    auto restore = [&]() -> hipError_t {
+        c->itt = itt0;
+        c->cur = cur0;
+        c->ring = ring0;
+        for (size_t n = 0; n < cb0.size(); ++n) c->trc[n].cbuf_valid = cb0[n];
+        return hipMemcpyAsync(c->state, snap, bytes, hipMemcpyDeviceToDevice, c->stream);
+    };
+    int rc = 0;
+    double best = 1e300;
+    int best_k = tile0;
+    if (hipMemcpyAsync(snap, c->state, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) rc = 3;
+    c->use_graph = false;   // time the kernels, not graph capture
+    for (int k = 0; k < kNumTiles && !rc; ++k) {
+        c->tile = k;
+        float ms = 0.f;
+        // untimed run first (first-launch costs), then the timed run from the same state
+        rc = c->fp32 ? step_many<float>(c, steps) : step_many<double>(c, steps);
+        if (rc || restore() != hipSuccess) { rc = rc ? rc : 3; break; }
+        if (hipEventRecord(c->ev0, c->stream) != hipSuccess) { rc = 3; break; }
+        rc = c->fp32 ? step_many<float>(c, steps) : step_many<double>(c, steps);
+        if (rc) break;
+        if (hipEventRecord(c->ev1, c->stream) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess ||
+            hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess || restore() != hipSuccess) {
+            rc = 3;
+            break;
+        }
+        if (times_ms) times_ms[k] = ms / steps;
+        if (ms < best) { best = ms; best_k = k; }
+    }
+    if (rc) (void)restore();
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(snap);
+    c->use_graph = graph0;
+    c->tile = rc ? tile0 : best_k;
+    drop_graphs(c);
+    if (rc) return rc == 3 ? fail(c, "tuning run failed: " + c->err, 3) : rc;
+    *best_shape = best_k;
+    return 0;
+}
+
+int swm_gpu_set_sor_check_every(swm_ctx* c, int32_t every) {
+    if (!c) return 1;
+    if (every < 1 || every > 64) return fail(c, "check cadence must be 1..64 sweeps", 1);
+    c->sor_every = every;
+    return 0;
+}
+
 int swm_gpu_velocity_potential(swm_ctx* c, double omega, double rel_tol, int64_t max_sweeps, int32_t fp32_sweeps,
                                int32_t reset, int64_t* iters, double* rel_res) {
     if (!c || !iters || !rel_res) return 1;
@@ -1260,13 +1341,16 @@ struct swm_batch {
     int nwork = 0, ntrc_max = 0;
     hipGraphExec_t graph = nullptr;
     bool use_graph = true;               // SWM_GPU_GRAPH=0: launch each step directly (profiling)
+    int shape = 0;                       // batch tile shape: kBatchShapes[shape]
     double last_ms = 0.0;
     std::string err;
 };
 
 namespace {
 
-constexpr int kBatchTx = 64, kBatchTy = 4;
+struct BatchShape { int tx, ty; };
+constexpr BatchShape kBatchShapes[] = {{64, 4}, {32, 8}};
+constexpr int kNumBatchShapes = 2;
 
 int bfail(swm_batch* b, const std::string& msg, int code = 2) {
     if (b) b->err = msg;
@@ -1327,8 +1411,9 @@ int batch_build_t(swm_batch* b) {
         m.s = (const Statics<T>*)c->statics;
         m.state = (T*)c->state;
         m.plane = c->plane;
-        m.ntx = (c->Nx + kBatchTx - 1) / kBatchTx;
-        m.ntile = m.ntx * ((c->Ny + kBatchTy - 1) / kBatchTy);
+        const BatchShape bs = kBatchShapes[b->shape];
+        m.ntx = (c->Nx + bs.tx - 1) / bs.tx;
+        m.ntile = m.ntx * ((c->Ny + bs.ty - 1) / bs.ty);
         m.wbase = (int)work;
         m.ntr = (int)c->trc.size();
         for (int t = 0; t < 3; ++t)
@@ -1363,16 +1448,22 @@ int batch_build_t(swm_batch* b) {
     return 0;
 }
 
-template <typename T, int V>
-void batch_launch_v(swm_batch* b) {
-    const int nmem = (int)b->mem.size();
+template <typename T, int V, int TX, int TY>
+void batch_launch_s(swm_batch* b) {
     dim3 grid(b->nwork), block(256);
     const swm::Member<T>* m = (const swm::Member<T>*)b->dmem;
     switch (b->ntrc_max) {
-    case 0: hipLaunchKernelGGL((swm::batch_kernel<T, V, 0, kBatchTx, kBatchTy>), grid, block, 0, b->stream, m, b->dstep, b->downer, b->nwork); break;
-    case 1: hipLaunchKernelGGL((swm::batch_kernel<T, V, 1, kBatchTx, kBatchTy>), grid, block, 0, b->stream, m, b->dstep, b->downer, b->nwork); break;
-    default: hipLaunchKernelGGL((swm::batch_kernel<T, V, 2, kBatchTx, kBatchTy>), grid, block, 0, b->stream, m, b->dstep, b->downer, b->nwork); break;
+    case 0: hipLaunchKernelGGL((swm::batch_kernel<T, V, 0, TX, TY>), grid, block, 0, b->stream, m, b->dstep, b->downer, b->nwork); break;
+    case 1: hipLaunchKernelGGL((swm::batch_kernel<T, V, 1, TX, TY>), grid, block, 0, b->stream, m, b->dstep, b->downer, b->nwork); break;
+    default: hipLaunchKernelGGL((swm::batch_kernel<T, V, 2, TX, TY>), grid, block, 0, b->stream, m, b->dstep, b->downer, b->nwork); break;
     }
+}
+
+template <typename T, int V>
+void batch_launch_v(swm_batch* b) {
+    const int nmem = (int)b->mem.size();
+    if (b->shape == 1) batch_launch_s<T, V, 32, 8>(b);
+    else batch_launch_s<T, V, 64, 4>(b);
     hipLaunchKernelGGL(swm::batch_advance, dim3((nmem + 255) / 256), dim3(256), 0, b->stream, b->dstep, nmem);
 }
 
@@ -1530,6 +1621,90 @@ int swm_gpu_batch_step(swm_batch* b, int64_t nsteps, int32_t* active) {
         for (const auto& h : b->hstep) na += h.remaining > 0;
         *active = na;
     }
+    return 0;
+}
+
+int swm_gpu_batch_set_shape(swm_batch* b, int32_t shape) {
+    if (!b) return 1;
+    if (shape < 0 || shape >= kNumBatchShapes) return bfail(b, "no such batch tile shape", 1);
+    HipLock lock(g_hip_mu);
+    if (int rc = batch_pull(b)) return rc;
+    b->shape = shape;
+    b->dirty = true;
+    return 0;
+}
+
+int swm_gpu_batch_tune(swm_batch* b, int32_t steps, int32_t* best_shape, double* times_ms) {
+    if (!b || !best_shape) return 1;
+    if (steps < 1) return bfail(b, "steps must be >= 1", 1);
+    if (b->mem.empty()) return bfail(b, "no members to tune on", 1);
+    std::vector<void*> snaps;
+    std::vector<size_t> sizes;
+    std::vector<swm::MemberStep> h0;
+    {
+        HipLock lock(g_hip_mu);
+        if (int rc = batch_pull(b)) return rc;
+        h0 = b->hstep;
+        for (swm_ctx* c : b->mem) {
+            const size_t bytes = (size_t)(swm::PL_TRC + swm::PL_PER_TRC * c->trc.size()) * c->plane * c->esz;
+            void* p = nullptr;
+            if (hipMalloc(&p, bytes) != hipSuccess) {
+                for (void* q : snaps) (void)hipFree(q);
+                return bfail(b, "out of device memory for the tuning snapshot", 3);
+            }
+            BHIPCHK(b, hipMemcpyAsync(p, c->state, bytes, hipMemcpyDeviceToDevice, b->stream));
+            snaps.push_back(p);
+            sizes.push_back(bytes);
+        }
+    }
+    const int shape0 = b->shape;
+    const bool graph0 = b->use_graph;
+    double best = 1e300;
+    int best_k = shape0, rc = 0;
+    b->use_graph = false;
+    auto reset = [&](int k) {
+        HipLock lock(g_hip_mu);
+        b->shape = k;
+        b->hstep = h0;
+        for (auto& h : b->hstep) h.remaining = h.remaining > 0 ? steps : 0;
+        b->on_device = false;
+        b->dirty = true;
+    };
+    auto restore = [&]() -> int {
+        HipLock lock(g_hip_mu);
+        for (size_t q = 0; q < snaps.size(); ++q)
+            if (hipMemcpyAsync(b->mem[q]->state, snaps[q], sizes[q], hipMemcpyDeviceToDevice, b->stream) != hipSuccess)
+                return 3;
+        return 0;
+    };
+    for (int k = 0; k < kNumBatchShapes && !rc; ++k) {
+        // untimed run first (first-launch costs), then the timed run from the same state
+        for (int pass = 0; pass < 2 && !rc; ++pass) {
+            reset(k);
+            if ((rc = swm_gpu_batch_step(b, steps, nullptr))) break;
+            rc = restore();
+        }
+        if (rc) break;
+        if (times_ms) times_ms[k] = b->last_ms / steps;
+        if (b->last_ms < best) { best = b->last_ms; best_k = k; }
+    }
+    HipLock lock(g_hip_mu);
+    (void)hipStreamSynchronize(b->stream);
+    for (void* q : snaps) (void)hipFree(q);
+    b->hstep = h0;
+    b->on_device = false;
+    for (size_t q = 0; q < b->mem.size(); ++q) {
+        swm_ctx* c = b->mem[q];
+        c->itt = h0[q].itt;
+        c->cur = h0[q].cur;
+        c->ring = h0[q].ring;
+        for (size_t n = 0; n < c->trc.size(); ++n) c->trc[n].cbuf_valid = ((h0[q].cbuf >> n) & 1) != 0;
+    }
+    b->use_graph = graph0;
+    b->shape = rc ? shape0 : best_k;
+    b->dirty = true;
+    if (rc) return rc;
+    *best_shape = best_k;
     return 0;
 }
 

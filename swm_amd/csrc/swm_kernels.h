@@ -1285,6 +1285,7 @@ struct SorArgs {
     double* partial;            // per-block partial sums of res^2 (or r^2)
     const int* done;            // set once converged: later sweeps return at once
     int residual_mode;          // 0: sum res^2 (reference), 1: sum (L chi - B)^2
+    int accumulate;             // 0: no convergence test after this sweep (no partial sums)
 };
 
 // Each thread updates one cell of the colour; row j holds the colour's cells
@@ -1333,6 +1334,7 @@ __global__ void __launch_bounds__(256) sor_sweep(const SorArgs<T> a) {
             }
         }
     }
+    if (!a.accumulate) return;   // uniform: no barrier is skipped by part of the block
     // fixed-order block reduction: wavefront butterfly then the 4 wave sums in order
     for (int s = 32; s >= 1; s >>= 1) acc += __shfl_xor(acc, s, 64);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;

@@ -34,7 +34,8 @@ EXPORTED = [
     "swm_gpu_last_timing", "swm_gpu_last_error", "swm_gpu_destroy",
     "swm_gpu_batch_create", "swm_gpu_batch_attach", "swm_gpu_batch_detach", "swm_gpu_batch_set_steps",
     "swm_gpu_batch_remaining", "swm_gpu_batch_step", "swm_gpu_batch_last_timing", "swm_gpu_batch_last_error",
-    "swm_gpu_batch_destroy",
+    "swm_gpu_batch_destroy", "swm_gpu_kernel_shape_count", "swm_gpu_set_kernel_shape", "swm_gpu_tune",
+    "swm_gpu_set_sor_check_every", "swm_gpu_batch_set_shape", "swm_gpu_batch_tune",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -126,6 +127,18 @@ def load_library(path: str = LIB_PATH):
     L.swm_gpu_batch_last_error.restype = C.c_char_p
     L.swm_gpu_batch_destroy.argtypes = [p]
     L.swm_gpu_batch_destroy.restype = None
+    L.swm_gpu_kernel_shape_count.argtypes = []
+    L.swm_gpu_kernel_shape_count.restype = C.c_int
+    L.swm_gpu_set_kernel_shape.argtypes = [p, C.c_int32]
+    L.swm_gpu_set_kernel_shape.restype = C.c_int
+    L.swm_gpu_tune.argtypes = [p, C.c_int32, C.POINTER(C.c_int32), _dp]
+    L.swm_gpu_tune.restype = C.c_int
+    L.swm_gpu_set_sor_check_every.argtypes = [p, C.c_int32]
+    L.swm_gpu_set_sor_check_every.restype = C.c_int
+    L.swm_gpu_batch_set_shape.argtypes = [p, C.c_int32]
+    L.swm_gpu_batch_set_shape.restype = C.c_int
+    L.swm_gpu_batch_tune.argtypes = [p, C.c_int32, C.POINTER(C.c_int32), _dp]
+    L.swm_gpu_batch_tune.restype = C.c_int
     L.swm_gpu_destroy.argtypes = [p]
     L.swm_gpu_destroy.restype = None
     _lib = L
@@ -302,6 +315,21 @@ class SwmGpu:
             self._check(rc, "swm_gpu_velocity_potential")
         return it.value, rr.value, rc == 0
 
+    def set_kernel_shape(self, shape: int):
+        self._check(self.L.swm_gpu_set_kernel_shape(self._h, int(shape)), "swm_gpu_set_kernel_shape")
+
+    def tune(self, steps: int = 20):
+        """Time every kernel shape from a snapshot (state restored exactly);
+        selects the fastest.  Returns (best_shape, per-step ms per shape)."""
+        n = self.L.swm_gpu_kernel_shape_count()
+        times = (C.c_double * n)()
+        best = C.c_int32()
+        self._check(self.L.swm_gpu_tune(self._h, int(steps), C.byref(best), C.cast(times, _dp)), "swm_gpu_tune")
+        return best.value, list(times)
+
+    def set_sor_check_every(self, every: int):
+        self._check(self.L.swm_gpu_set_sor_check_every(self._h, int(every)), "swm_gpu_set_sor_check_every")
+
     def sums(self, tracer: int = -1):
         """FP64 compensated (volume, energy, tracer inventory) sums, cos-weighted
         over ocean eta points (swm_gpu_sums)."""
@@ -372,6 +400,18 @@ class SwmBatch:
         a = C.c_int32()
         self._check(self.L.swm_gpu_batch_step(self._h, int(nsteps), C.byref(a)), "swm_gpu_batch_step")
         return a.value
+
+    def set_shape(self, shape: int):
+        self._check(self.L.swm_gpu_batch_set_shape(self._h, int(shape)), "swm_gpu_batch_set_shape")
+
+    def tune(self, steps: int = 10):
+        """Time each batch tile shape from a snapshot of every member (restored
+        exactly); selects the fastest.  Returns (best_shape, per-step ms)."""
+        times = (C.c_double * 2)()
+        best = C.c_int32()
+        self._check(self.L.swm_gpu_batch_tune(self._h, int(steps), C.byref(best), C.cast(times, _dp)),
+                    "swm_gpu_batch_tune")
+        return best.value, list(times)
 
     def last_ms(self) -> float:
         t = C.c_double()
