@@ -146,6 +146,17 @@ int swm_gpu_sor(swm_ctx *c, const double *B, double *chi, double epsilon, int32_
 int swm_gpu_sor_fixed(swm_ctx *c, const double *B, double *chi, double omega, double rel_tol,
                       int64_t max_sweeps, int32_t fp32_sweeps, int64_t *iters, double *rel_res);
 
+/* Averaging diag tasks kept on the device (processTask / addDataToTaskBuffer,
+ * src/diagTask.f90:266-340): a registered buffer receives buffer += dt *
+ * field**p (p = 2 for square averaging) in FP64 after every step, including
+ * steps replayed from graphs, so averaging tasks need no per-step copies.
+ * Fields: SWM_U, SWM_V, SWM_ETA, CH (tracer_id).  avg_read returns
+ * buffer / bufferCount (bufferCount = the accumulated dt) and resets both when
+ * reset != 0, as the reference does after writing an average.  Contexts with
+ * averages are stepped alone (not in a batch). */
+int swm_gpu_avg_add(swm_ctx *c, const char *name, int32_t tracer_id, int32_t square, int32_t *avg_id);
+int swm_gpu_avg_read(swm_ctx *c, int32_t avg_id, double *host_out, double *weight, int32_t reset);
+
 /* Tuning knobs and the autotuner.  No counterpart in the reference (its loops
  * have no launch shapes); they serve callers that pick kernel shapes for their
  * grids.  Shapes 0..swm_gpu_kernel_shape_count()-1: 0 = 64x4 LDS tiles

@@ -102,6 +102,14 @@ struct swm_ctx {
     double* norm_buf = nullptr;  // ||B|| partials and result of the device-resident SOR
     bool chi_valid[2] = {};      // sor_buf[w][0] holds a solution to warm-start from
     int sor_every = 1;           // fixed-omega mode: convergence test every sor_every sweeps
+    struct Avg {                 // averaging buffer of a diag task
+        int field;               // 0 u, 1 v, 2 eta, 3 CH of tracer `tracer`
+        int tracer;
+        int square;
+        double* buf;             // FP64, pitched like the fields
+        double weight;           // bufferCount
+    };
+    std::vector<Avg> avgs;
     int chi_last = -1;           // precision slot of the last solution
     double* ce64 = nullptr;      // cos(lat_eta) rows in FP64
     std::vector<double> ce_host;
@@ -572,6 +580,15 @@ int step_one(swm_ctx* c) {
     c->cur = 1 - c->cur;
     c->ring = (c->ring + 1) % 3;
     for (auto& tr : c->trc) tr.cbuf_valid = false;
+    // averaging diag tasks: add dt * field**p of the new N0 (Diag after model_advance)
+    for (auto& av : c->avgs) {
+        const int pl = av.field == 0 ? swm::PL_U + c->cur : av.field == 1 ? swm::PL_V + c->cur
+                     : av.field == 2 ? swm::PL_E + c->cur : swm::PL_TRC + swm::PL_PER_TRC * av.tracer + c->cur;
+        const long n = (long)c->Nx * c->Ny;
+        hipLaunchKernelGGL(swm::avg_accumulate<T>, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->Nx, c->Ny, c->P,
+                           (const T*)plane_ptr(c, pl), av.buf, c->d.dt, av.square);
+        av.weight += c->d.dt;
+    }
     return 0;
 }
 
@@ -594,6 +611,8 @@ int step_many(swm_ctx* c, int64_t nsteps) {
             const int key = c->cur * 3 + c->ring;
             if (!c->graphs[key]) {
                 HipLock lock(g_hip_mu);
+                std::vector<double> w0;
+                for (auto& av : c->avgs) w0.push_back(av.weight);
                 hipGraph_t g = nullptr;
                 HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 int rc = 0;
@@ -605,10 +624,13 @@ int step_many(swm_ctx* c, int64_t nsteps) {
                 (void)hipGraphDestroy(g);
                 if (e != hipSuccess) return fail(c, std::string("graph instantiate: ") + hipGetErrorString(e), 3);
                 c->itt -= 6;   // capture recorded the six steps without running them
+                for (size_t q = 0; q < w0.size(); ++q) c->avgs[q].weight = w0[q];
             }
             HIPCHK(c, hipGraphLaunch(c->graphs[key], c->stream));
             c->itt += 6;
             done += 6;
+            for (auto& av : c->avgs)   // the graph holds their accumulations; bufferCount adds dt per step
+                for (int q = 0; q < 6; ++q) av.weight += c->d.dt;
         } else {
             int rc = step_one<T>(c);
             if (rc) return rc;
@@ -1222,6 +1244,51 @@ int swm_gpu_tune(swm_ctx* c, int32_t steps, int32_t* best_shape, double* times_m
     return 0;
 }
 
+int swm_gpu_avg_add(swm_ctx* c, const char* name, int32_t tracer_id, int32_t square, int32_t* avg_id) {
+    if (!c || !name || !avg_id) return 1;
+    HipLock lock(g_hip_mu);
+    if (int rc = member_refresh(c)) return rc;
+    if (c->batch) return fail(c, "context is attached to a batch; register averages before attaching", 1);
+    std::string s(name);
+    for (auto& ch : s) ch = (char)toupper((unsigned char)ch);
+    int field = -1;
+    if (s == "SWM_U" || s == "U") field = 0;
+    else if (s == "SWM_V" || s == "V") field = 1;
+    else if (s == "SWM_ETA" || s == "ETA") field = 2;
+    else if (s == "CH") field = 3;
+    if (field < 0) return fail(c, "averages are kept for SWM_U, SWM_V, SWM_ETA and CH", 1);
+    if (field == 3 && (tracer_id < 0 || tracer_id >= (int32_t)c->trc.size())) return fail(c, "no such tracer", 1);
+    swm_ctx::Avg av{field, field == 3 ? tracer_id : -1, square ? 1 : 0, nullptr, 0.0};
+    const size_t bytes = (size_t)c->P * c->Ny * sizeof(double);
+    if (!(av.buf = (double*)dalloc(c, bytes))) return fail(c, "out of device memory", 3);
+    HIPCHK(c, hipMemsetAsync(av.buf, 0, bytes, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);   // replayed steps now also accumulate this buffer
+    c->avgs.push_back(av);
+    *avg_id = (int32_t)c->avgs.size() - 1;
+    return 0;
+}
+
+int swm_gpu_avg_read(swm_ctx* c, int32_t avg_id, double* host_out, double* weight, int32_t reset) {
+    if (!c || !host_out) return 1;
+    HipLock lock(g_hip_mu);
+    if (int rc = member_refresh(c)) return rc;
+    if (avg_id < 0 || avg_id >= (int32_t)c->avgs.size()) return fail(c, "no such average", 1);
+    swm_ctx::Avg& av = c->avgs[avg_id];
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->Nx * c->Ny;
+    HIPCHK(c, hipMemcpy2D(host_out, c->Nx * 8, av.buf, c->P * 8, c->Nx * 8, c->Ny, hipMemcpyDeviceToHost));
+    // task%buffer = task%buffer / task%bufferCount
+    for (size_t k = 0; k < n; ++k) host_out[k] = host_out[k] / av.weight;
+    if (weight) *weight = av.weight;
+    if (reset) {
+        HIPCHK(c, hipMemsetAsync(av.buf, 0, (size_t)c->P * c->Ny * sizeof(double), c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        av.weight = 0.0;
+    }
+    return 0;
+}
+
 int swm_gpu_set_sor_check_every(swm_ctx* c, int32_t every) {
     if (!c) return 1;
     if (every < 1 || every > 64) return fail(c, "check cadence must be 1..64 sweeps", 1);
@@ -1509,6 +1576,7 @@ int swm_gpu_batch_attach(swm_batch* b, swm_ctx* c, int64_t nsteps) {
     if (c->batch) return bfail(b, "context is already attached to a batch", 1);
     if (nsteps < 0) return bfail(b, "nsteps must be >= 0", 1);
     if (c->has_osc) return bfail(b, "oscillating forcing needs per-step host phases; step such a context alone", 4);
+    if (!c->avgs.empty()) return bfail(b, "averaging buffers are kept by contexts stepped alone", 4);
     if (b->fp32 >= 0 && (b->fp32 != (int)c->fp32 || b->variant != c->d.variant))
         return bfail(b, "every member of a batch has the same precision and variant", 4);
     if (int rc = batch_pull(b)) return rc;

@@ -1427,6 +1427,21 @@ sq_norm_final(const double* partial, int nblk, double* out) {
 }
 
 // ---------------------------------------------------------------------------
+// Averaging buffer of a diag task (addDataToTaskBuffer, src/diagTask.f90:318-340):
+// buf = buf + w * f**p, p = 1 or 2, in FP64 whatever the field precision.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+avg_accumulate(int Nx, int Ny, int P, const T* f, double* buf, double w, int square) {
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (long)Nx * Ny) return;
+    const int j = (int)(idx / Nx), i = (int)(idx - (long)j * Nx);
+    const size_t off = (size_t)j * P + i;
+    const double x = (double)f[off];
+    buf[off] = buf[off] + w * (square ? x * x : x);
+}
+
+// ---------------------------------------------------------------------------
 // Conservation sums in FP64 with compensated (Neumaier) accumulation, for the
 // FP32 storage path as well as the FP64 one.  Over ocean eta points, weighted
 // by cos(lat_eta):

@@ -35,7 +35,8 @@ EXPORTED = [
     "swm_gpu_batch_create", "swm_gpu_batch_attach", "swm_gpu_batch_detach", "swm_gpu_batch_set_steps",
     "swm_gpu_batch_remaining", "swm_gpu_batch_step", "swm_gpu_batch_last_timing", "swm_gpu_batch_last_error",
     "swm_gpu_batch_destroy", "swm_gpu_kernel_shape_count", "swm_gpu_set_kernel_shape", "swm_gpu_tune",
-    "swm_gpu_set_sor_check_every", "swm_gpu_batch_set_shape", "swm_gpu_batch_tune",
+    "swm_gpu_set_sor_check_every", "swm_gpu_batch_set_shape", "swm_gpu_batch_tune", "swm_gpu_avg_add",
+    "swm_gpu_avg_read",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -139,6 +140,10 @@ def load_library(path: str = LIB_PATH):
     L.swm_gpu_batch_set_shape.restype = C.c_int
     L.swm_gpu_batch_tune.argtypes = [p, C.c_int32, C.POINTER(C.c_int32), _dp]
     L.swm_gpu_batch_tune.restype = C.c_int
+    L.swm_gpu_avg_add.argtypes = [p, C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
+    L.swm_gpu_avg_add.restype = C.c_int
+    L.swm_gpu_avg_read.argtypes = [p, C.c_int32, _dp, _dp, C.c_int32]
+    L.swm_gpu_avg_read.restype = C.c_int
     L.swm_gpu_destroy.argtypes = [p]
     L.swm_gpu_destroy.restype = None
     _lib = L
@@ -314,6 +319,21 @@ class SwmGpu:
         if rc not in (0, 1):
             self._check(rc, "swm_gpu_velocity_potential")
         return it.value, rr.value, rc == 0
+
+    def avg_add(self, name: str, tracer: int = -1, square: bool = False) -> int:
+        """Register an averaging diag buffer (dt * field**p added after every step)."""
+        aid = C.c_int32()
+        self._check(self.L.swm_gpu_avg_add(self._h, name.encode(), int(tracer), int(bool(square)), C.byref(aid)),
+                    "swm_gpu_avg_add")
+        return aid.value
+
+    def avg_read(self, avg_id: int, reset: bool = True):
+        """(buffer / bufferCount, bufferCount); resets the buffer when asked."""
+        out = np.empty((self.s.Ny, self.s.Nx), dtype=np.float64)
+        w = C.c_double()
+        self._check(self.L.swm_gpu_avg_read(self._h, int(avg_id), out.ctypes.data_as(_dp), C.byref(w),
+                                            int(bool(reset))), "swm_gpu_avg_read")
+        return out, w.value
 
     def set_kernel_shape(self, shape: int):
         self._check(self.L.swm_gpu_set_kernel_shape(self._h, int(shape)), "swm_gpu_set_kernel_shape")

@@ -279,3 +279,33 @@ def test_velocity_potential_device_resident(gpu_lib):
     it_z, _, ok_z = g.velocity_potential(1.9, 1e-10, 5000, reset=True)
     assert ok_w and ok_z and it_w < it_z
     g.close()
+
+
+def test_device_averaging_matches_reference_order(gpu_lib):
+    """Averaging diag tasks on the device: buffer = buffer + dt*f (and dt*f*f for
+    square averaging) after every step, divided by the accumulated dt, equals
+    the same sums over the oracle's states bit for bit, across graph replays
+    and a reset."""
+    s = configs.c2_nonlinear(seed=6, N=80, dt=9.7)   # dt not a binary fraction: weights must add in order
+    g = gpu.SwmGpu(s).init_from_setup()
+    o = oracle.Oracle(s)
+    a_eta = g.avg_add("SWM_ETA")
+    a_u2 = g.avg_add("SWM_U", square=True)
+    a_ch = g.avg_add("CH", tracer=0)
+    for nsteps, chunks in ((13, (13,)), (20, (7, 13))):
+        buf = {k: np.zeros(s.shape) for k in ("eta", "u2", "ch")}
+        cnt = 0.0
+        for _ in range(nsteps):
+            o.step(1)
+            buf["eta"] = buf["eta"] + s.dt * o.get("eta")
+            u = o.get("u")
+            buf["u2"] = buf["u2"] + s.dt * (u * u)
+            buf["ch"] = buf["ch"] + s.dt * o.get("CH", 0)
+            cnt = cnt + s.dt
+        for n in chunks:
+            g.step(n)
+        for aid, key in ((a_eta, "eta"), (a_u2, "u2"), (a_ch, "ch")):
+            got, w = g.avg_read(aid, reset=True)
+            assert w == cnt
+            assert same(got, buf[key] / cnt), key
+    g.close()
