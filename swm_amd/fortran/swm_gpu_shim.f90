@@ -91,6 +91,41 @@ module swm_gpu_shim
       integer(c_int64_t), intent(out) :: iters
       real(c_double), intent(out) :: anorm
     end function
+    integer(c_int) function swm_gpu_velocity_potential(ctx, omega, rel_tol, max_sweeps, fp32_sweeps, reset, &
+        iters, rel_res) bind(C, name="swm_gpu_velocity_potential")
+      import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), value :: omega, rel_tol
+      integer(c_int64_t), value :: max_sweeps
+      integer(c_int32_t), value :: fp32_sweeps, reset
+      integer(c_int64_t), intent(out) :: iters
+      real(c_double), intent(out) :: rel_res
+    end function
+    integer(c_int) function swm_gpu_avg_add(ctx, name, tracer_id, square, avg_id) bind(C, name="swm_gpu_avg_add")
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: name
+      integer(c_int32_t), value :: tracer_id, square
+      integer(c_int32_t), intent(out) :: avg_id
+    end function
+    integer(c_int) function swm_gpu_avg_read(ctx, avg_id, out, weight, reset) bind(C, name="swm_gpu_avg_read")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx, out
+      integer(c_int32_t), value :: avg_id, reset
+      real(c_double), intent(out) :: weight
+    end function
+    integer(c_int) function swm_gpu_sums(ctx, tracer_id, out) bind(C, name="swm_gpu_sums")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tracer_id
+      real(c_double), dimension(3), intent(out) :: out
+    end function
+    integer(c_int) function swm_gpu_tune(ctx, steps, best_shape, times_ms) bind(C, name="swm_gpu_tune")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx, times_ms
+      integer(c_int32_t), value :: steps
+      integer(c_int32_t), intent(out) :: best_shape
+    end function
     type(c_ptr) function swm_gpu_last_error(ctx) bind(C, name="swm_gpu_last_error")
       import :: c_ptr
       type(c_ptr), value :: ctx
@@ -102,8 +137,9 @@ module swm_gpu_shim
   end interface
 
   public :: swm_gpu_create, swm_gpu_upload_state, swm_gpu_add_tracer, swm_gpu_step, swm_gpu_itt, &
-            swm_gpu_sync, swm_gpu_sor, swm_gpu_last_error, swm_gpu_destroy
-  public :: gpu_check, gpu_sync_field, gpu_main_ElSolv_SOR
+            swm_gpu_sync, swm_gpu_sor, swm_gpu_velocity_potential, swm_gpu_avg_add, swm_gpu_avg_read, &
+            swm_gpu_sums, swm_gpu_tune, swm_gpu_last_error, swm_gpu_destroy
+  public :: gpu_check, gpu_sync_field, gpu_main_ElSolv_SOR, gpu_avg_register, gpu_avg_write
 
 contains
 
@@ -152,5 +188,26 @@ contains
       call gpu_check(ctx, rc, "swm_gpu_sor")
     end if
   end subroutine gpu_main_ElSolv_SOR
+
+  !> Averaging diag task kept on the device (diagTask.f90 type "A"): register once,
+  !! then read (and reset) at each output step instead of adding every step on the host.
+  subroutine gpu_avg_register(ctx, name, tracer_id, square, avg_id)
+    type(c_ptr), intent(in) :: ctx
+    character(*), intent(in) :: name
+    integer, intent(in) :: tracer_id
+    logical, intent(in) :: square
+    integer(c_int32_t), intent(out) :: avg_id
+    call gpu_check(ctx, swm_gpu_avg_add(ctx, trim(name)//c_null_char, int(tracer_id, c_int32_t), &
+                   merge(1_c_int32_t, 0_c_int32_t, square), avg_id), "swm_gpu_avg_add "//trim(name))
+  end subroutine gpu_avg_register
+
+  !> task%buffer / task%bufferCount of an averaging task, buffer reset afterwards.
+  subroutine gpu_avg_write(ctx, avg_id, field)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int32_t), intent(in) :: avg_id
+    real(c_double), dimension(:, :), target, contiguous, intent(inout) :: field
+    real(c_double) :: weight
+    call gpu_check(ctx, swm_gpu_avg_read(ctx, avg_id, c_loc(field), weight, 1_c_int32_t), "swm_gpu_avg_read")
+  end subroutine gpu_avg_write
 
 end module swm_gpu_shim

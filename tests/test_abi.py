@@ -57,3 +57,21 @@ def test_missing_library_fails_loudly(tmp_path):
             gpu.load_library(str(tmp_path / "absent.so"))
         finally:
             gpu._lib = None
+
+
+def test_fortran_shim_compiles(tmp_path):
+    """The ISO_C_BINDING module a maintainer adds to the reference build
+    compiles with the image's Fortran compiler, and every bind(C) name it
+    declares is exported by the library."""
+    import shutil
+    import subprocess
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        pytest.skip("no Fortran compiler in this image")
+    src = os.path.join(ROOT, "swm_amd", "fortran", "swm_gpu_shim.f90")
+    r = subprocess.run([fc, "-c", src, "-o", str(tmp_path / "shim.o"), "-module-dir", str(tmp_path)],
+                       capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    from swm_amd import gpu
+    names = set(re.findall(r'bind\(C, name="([a-z_]+)"\)', open(src).read()))
+    assert names and names <= set(gpu.EXPORTED)
