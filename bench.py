@@ -7,6 +7,10 @@ A step is one pass of SWM_timestep + TRC_timestep + model_advance over the
 whole grid, fields resident in HBM.
 
 value = cell-updates/s = Nx*Ny*K / (device time of the K timed steps).
+At the stated dt = 30 s the fields overflow within ~40 steps (AB3 is unstable
+for this grid's fastest gravity waves), so the line also carries the same
+timing at dt = 18 s, the largest stable whole-second step, with finite fields
+(config.stable_dt_run).
 roofline.achieved = algorithmic bytes per step / average step-kernel time,
 where the algorithmic bytes per cell-step are
   state u, v, eta read + write         6 x 8 = 48
@@ -37,6 +41,11 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/s + % HBM roofline; served-ensemble p50/p99 latency vs plain"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"f64": 121 + 40, "f32": 61 + 20}
+# AB3 is stable for gravity waves while omega_max*dt < ~0.72, with
+# omega_max = 2*sqrt(2)*sqrt(G*H)/dx = 0.0396 1/s on the 1024 x 1024 basin
+# (H = 500 m, dx = 5 km): dt <= 18 s.  The stated dt = 30 s overflows within
+# ~40 steps (amplification 2.09 per step), in the CPU restatement as on the GPU.
+STABLE_DT = 18.0
 
 
 def read_traffic(precision: str):
@@ -334,6 +343,19 @@ def main(argv=None):
     import numpy as np
     finite = bool(np.isfinite(eta).all())
     g.close()
+    stable = None
+    if args.dt > STABLE_DT:
+        # The same workload at the largest whole-second dt inside AB3's stability
+        # region: the step time must not depend on whether the fields overflowed.
+        s2 = configs.c2_nonlinear(seed=0, N=args.n, dt=STABLE_DT, nsteps=args.warmup + args.steps)
+        g2 = gpu.SwmGpu(s2, precision=prec).init_from_setup()
+        g2.step(args.warmup)
+        g2.step(args.steps)
+        ms2, _ = g2.last_timing()
+        stable = {"dt": STABLE_DT, "ms_per_step": ms2 / args.steps,
+                  "value": s2.Nx * s2.Ny * args.steps / (ms2 * 1e-3),
+                  "fields_finite_at_end": bool(np.isfinite(g2.get("SWM_ETA")).all())}
+        g2.close()
 
     copy_gbs, _ = copy_bandwidth()
     cells = s.Nx * s.Ny
@@ -357,6 +379,7 @@ def main(argv=None):
             "workload": f"nonlinear beta-plane basin {s.Nx}x{s.Ny}, 1 tracer, 5% islands, AB3 dt={args.dt:g} s",
             "Nx": s.Nx, "Ny": s.Ny, "dt": args.dt, "tracers": 1, "variant": "FULLY_NONLINEAR+LATERAL_MIXING",
             "fields_finite_at_end": finite, "wall_s": wall,
+            "stable_dt_run": stable,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

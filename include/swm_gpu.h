@@ -160,7 +160,8 @@ int swm_gpu_avg_read(swm_ctx *c, int32_t avg_id, double *host_out, double *weigh
 /* Tuning knobs and the autotuner.  No counterpart in the reference (its loops
  * have no launch shapes); they serve callers that pick kernel shapes for their
  * grids.  Shapes 0..swm_gpu_kernel_shape_count()-1: 0 = 64x4 LDS tiles
- * (default), 1 = 32x8, 2 = 64x8, 3 = 128x4, 4 = 32x16, 5..8 = marching
+ * (default for FP64 grids of 3072^2 cells and more), 1 = 32x8 (default
+ * otherwise), 2 = 64x8, 3 = 128x4, 4 = 32x16, 5..8 = marching
  * wavefronts (automatic, 8, 16, 4 rows), 9, 10 = persistent 64x4, 32x8 tiles;
  * every shape gives the same FP64 results bit for bit.  swm_gpu_tune times
  * `steps` steps of every shape from a device snapshot of the state, restores
@@ -225,7 +226,7 @@ int swm_gpu_batch_detach(swm_batch *b, swm_ctx *c);
 int swm_gpu_batch_set_steps(swm_batch *b, swm_ctx *c, int64_t nsteps);
 int swm_gpu_batch_remaining(swm_batch *b, const swm_ctx *c, int64_t *remaining);
 int swm_gpu_batch_step(swm_batch *b, int64_t nsteps, int32_t *active);
-/* batch tile shape: 0 = 64x4 (default), 1 = 32x8; batch_tune times `steps`
+/* batch tile shape: 0 = 64x4, 1 = 32x8 (default); batch_tune times `steps`
  * steps of each from a device snapshot of every member, restores the members
  * exactly, selects the fastest (times_ms: 2 per-step times, may be NULL). */
 int swm_gpu_batch_set_shape(swm_batch *b, int32_t shape);

@@ -719,6 +719,11 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
     c->Ny = d->Ny;
     c->fp32 = d->precision == SWM_FP32;
     c->esz = c->fp32 ? 4 : 8;
+    // Default shape from measured step times on MI355X: 32x8 tiles carry less
+    // halo per output cell and win in FP32 at every size and in FP64 up to
+    // about 3072^2 cells; past that the FP64 kernel is HBM-bound and the
+    // longer 512-byte rows of 64x4 tiles win.
+    c->tile = (c->fp32 || (long long)d->Nx * d->Ny < 3072LL * 3072LL) ? 1 : 0;
     if (const char* e = getenv("SWM_GPU_TILE")) c->tile = std::max(0, std::min(kNumTiles - 1, atoi(e)));
     if (const char* e = getenv("SWM_GPU_GRAPH")) c->use_graph = atoi(e) != 0;
     const int align = c->fp32 ? 32 : 16;   // 128-byte aligned rows
@@ -1408,7 +1413,8 @@ struct swm_batch {
     int nwork = 0, ntrc_max = 0;
     hipGraphExec_t graph = nullptr;
     bool use_graph = true;               // SWM_GPU_GRAPH=0: launch each step directly (profiling)
-    int shape = 0;                       // batch tile shape: kBatchShapes[shape]
+    int shape = 1;                       // batch tile shape: kBatchShapes[shape] (32x8: members are
+                                         // small grids, where 32x8 tiles win)
     double last_ms = 0.0;
     std::string err;
 };
