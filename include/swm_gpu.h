@@ -46,6 +46,9 @@ enum swm_flags {
 enum swm_precision { SWM_FP64 = 0, SWM_FP32 = 1 };
 enum swm_grid_id { SWM_GRID_H = 0, SWM_GRID_U = 1, SWM_GRID_V = 2, SWM_GRID_ETA = 3 };
 enum swm_component { SWM_COMP_X = 0, SWM_COMP_Y = 1, SWM_COMP_ETA = 2 };
+/* swm_forcing_nl forcingtype (src/swm_forcing_module.f90:223-243) of a
+ * time-interpolated stream */
+enum swm_stream_kind { SWM_STREAM_WIND = 0, SWM_STREAM_CUSTOM = 1, SWM_STREAM_HEATING = 2 };
 
 /* grid_t, src/grid_module.f90:9-20 (H, ocean, cos_lat, tan_lat, f) */
 typedef struct swm_grid_desc {
@@ -110,6 +113,23 @@ int swm_gpu_upload_state(swm_ctx *c, const double *u, const double *v, const dou
 /* Replaces TRC_add_to_list (src/tracer_vars.f90:124-185) for one tracer.
  * Tracers are stepped in the order they were added (TRC_timestep list order). */
 int swm_gpu_add_tracer(swm_ctx *c, const swm_tracer_desc *t, int32_t *tracer_id);
+
+/* Replaces SWM_forcing_initStream (src/swm_forcing_module.f90:593-640) for a
+ * dataset read through a memoryChunk (src/memchunk_module.f90:77-115): nrec
+ * (Nx,Ny) records back to back, spaced rec_dt model seconds (getDt,
+ * memchunk_module.f90:252-273), record 1 at model time 0 and the dataset
+ * repeating after its last record; chunk_size is the namelist chunksize
+ * (SWM_DEF_FORCING_CHUNKSIZE = 100).  Every step adds, on the component's
+ * ocean points, the dataset interpolated in time as getChunkData(itt*dt)
+ * does (214-243): kind WIND as [TAU_SCALE*]tau/(RHO0*D_face) (divided by RHO0
+ * alone unless BAROTROPIC or FULLY_NONLINEAR; components X, Y), CUSTOM as is
+ * (X, Y, ETA), HEATING as is (ETA).  Streams are processed after the
+ * descriptor's windstress and oscillating streams, in the order added (the
+ * reference's list order when the namelist lists them last).  All records stay
+ * resident on the device (nrec planes).  At most 4 streams; a context with
+ * streams is stepped alone (not in a batch) and without graph replay. */
+int swm_gpu_add_forcing_stream(swm_ctx *c, int32_t kind, int32_t component, int32_t nrec, double rec_dt,
+                               int32_t chunk_size, const double *records, int32_t *stream_id);
 
 /* nsteps iterations of the time loop body of src/model.f90:159-196 without Diag:
  * SWM_timestep (src/swm_module.f90:71-74), TRC_timestep

@@ -46,6 +46,9 @@ class _Desc(C.Structure):
         ("kappa_h", C.c_double * MAX_TRACERS),
         ("trc_C", _dp * MAX_TRACERS), ("trc_C0", _dp * MAX_TRACERS),
         ("trc_gamma_C", _dp * MAX_TRACERS), ("trc_cons", _dp * MAX_TRACERS),
+        ("n_tdf", C.c_int32), ("tdf_kind", C.c_int32 * MAX_STREAMS), ("tdf_comp", C.c_int32 * MAX_STREAMS),
+        ("tdf_nrec", C.c_int32 * MAX_STREAMS), ("tdf_chunk", C.c_int32 * MAX_STREAMS),
+        ("tdf_dt", C.c_double * MAX_STREAMS), ("tdf_rec", _dp * MAX_STREAMS),
     ]
 
 
@@ -131,6 +134,13 @@ class Oracle:
             d.osc_re[n] = _ptr(re, k)
             d.osc_im[n] = _ptr(im, k)
             d.osc_omega[n] = om
+        streams = getattr(s, "streams", [])
+        assert len(streams) <= MAX_STREAMS
+        d.n_tdf = len(streams)
+        for n, st in enumerate(streams):
+            d.tdf_kind[n], d.tdf_comp[n] = st.kind, st.comp
+            d.tdf_nrec[n], d.tdf_chunk[n], d.tdf_dt[n] = st.records.shape[0], st.chunk_size, st.rec_dt
+            d.tdf_rec[n] = _ptr(np.ascontiguousarray(st.records, np.float64), k)
         d.n_tracer = len(s.tracers)
         for n, t in enumerate(s.tracers):
             d.kappa_h[n] = t["kappa_h"]

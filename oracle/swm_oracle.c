@@ -33,6 +33,18 @@ typedef struct {
     double *C, *C0, *gamma_C, *cons, *impl, *uhc, *vhc, *diff, *forcing;
 } tracer_t;
 
+/* memoryChunk, memchunk_module.f90:37-50: a buffer of chunkSize records of a
+ * dataset with its time axis.  The dataset is held in memory (file). */
+typedef struct memchunk {
+    const double *file;     /* nrec records of n doubles */
+    size_t n;
+    int64_t nrec, chunkSize, fileRec, chunkCounter, nFpC;
+    int isInitialised, isPersistent, isConstant;
+    double dt, tOffset;
+    double *var;            /* chunkSize records */
+    double *time;           /* chunkSize */
+} memchunk;
+
 struct orc_model {
     orc_desc d;
     int Nx, Ny;
@@ -60,6 +72,9 @@ struct orc_model {
     int ntrc;
     tracer_t trc[ORC_MAX_TRACERS];
     double *TRC_coeff;            /* (12, Nx, Ny), tracer_module.f90:42 */
+    /* memory chunks of the time-interpolated forcing streams */
+    memchunk mc[ORC_MAX_STREAMS];
+    double *tdf_buf;
     /* SOR, ElSolv_SOR.f90:25-32 */
     double *sor_c;                /* (Nx, Ny, 5) */
     int n_oe[2];
@@ -535,6 +550,80 @@ static void timestep_advance(orc_model *m) {
     computeMassFluxes(m);
 }
 
+/* getVarChunkFromDisk, memchunk_module.f90:181-207: len records from record
+ * *nstart (1-based) into var, wrapping to the start of the dataset. */
+static void mc_get_var_chunk(memchunk *mc, double *var, int64_t *nstart, int64_t len) {
+    if (*nstart > mc->nrec) *nstart = *nstart - mc->nFpC * mc->nrec;
+    if (*nstart == 0) *nstart = mc->nrec;
+    int64_t nend = *nstart + len - 1 < mc->nrec ? *nstart + len - 1 : mc->nrec;
+    int64_t len2 = (nend - *nstart > 0 ? nend - *nstart : 0) + 1;
+    memcpy(var, mc->file + (size_t)(*nstart - 1) * mc->n, (size_t)len2 * mc->n * sizeof(double));
+    if (len2 < len) {
+        int64_t nstart2 = 1;
+        mc_get_var_chunk(mc, var + (size_t)len2 * mc->n, &nstart2, len - len2);
+    }
+}
+
+/* getChunkFromDisk, memchunk_module.f90:149-170 */
+static void mc_get_chunk(memchunk *mc) {
+    if (mc->isInitialised) {
+        mc->fileRec = mc->fileRec + mc->chunkSize - 1;
+        mc->tOffset = mc->tOffset - mc->dt;
+    }
+    if (!mc->isPersistent || !mc->isInitialised) mc_get_var_chunk(mc, mc->var, &mc->fileRec, mc->chunkSize);
+    if (!mc->isConstant) {
+        for (int64_t i = 0; i < mc->chunkSize; ++i) {
+            mc->time[i] = mc->tOffset;
+            mc->tOffset = mc->tOffset + mc->dt;
+        }
+    }
+    mc->chunkCounter = 1;
+}
+
+/* initMemChunk, memchunk_module.f90:77-115 (the dataset's getDt is passed in) */
+static void mc_init(memchunk *mc, const double *file, size_t n, int64_t nrec, int64_t chunkSize, double dt) {
+    memset(mc, 0, sizeof(*mc));
+    mc->file = file;
+    mc->n = n;
+    mc->nrec = nrec;
+    mc->fileRec = 1;
+    mc->chunkCounter = 1;
+    mc->chunkSize = chunkSize;
+    if (nrec <= 1) {
+        mc->isPersistent = 1;
+        mc->isConstant = 1;
+        mc->chunkSize = 1;
+    } else if (chunkSize >= nrec + 1) {
+        mc->isPersistent = 1;
+        mc->chunkSize = nrec + 1;
+    }
+    mc->dt = mc->isConstant ? 0.0 : dt;
+    mc->tOffset = 0.0;
+    mc->var = zalloc((size_t)mc->chunkSize * n);
+    mc->time = zalloc((size_t)mc->chunkSize);
+    mc_get_chunk(mc);
+    int64_t q = (int64_t)floorf((float)mc->chunkSize / (float)nrec);   /* floor(REAL(chunkSize)/nrec) */
+    mc->nFpC = q > 1 ? q : 1;
+    mc->isInitialised = 1;
+}
+
+/* getChunkData, memchunk_module.f90:214-243, with interp1D_array2D
+ * (calc_lib.f90:386-397) between the bracketing records. */
+static void mc_get(memchunk *mc, double t, double *out) {
+    if (mc->isConstant) { memcpy(out, mc->var, mc->n * sizeof(double)); return; }
+    if (mc->chunkCounter >= mc->chunkSize) mc_get_chunk(mc);
+    while (mc->time[mc->chunkCounter] < t) {          /* time(chunkCounter+1), 1-based */
+        mc->chunkCounter = mc->chunkCounter + 1;
+        if (mc->chunkCounter >= mc->chunkSize) mc_get_chunk(mc);
+    }
+    const double *y0 = mc->var + (size_t)(mc->chunkCounter - 1) * mc->n;
+    const double x0 = mc->time[mc->chunkCounter - 1];
+    if (t == x0) { memcpy(out, y0, mc->n * sizeof(double)); return; }
+    const double *y1 = y0 + mc->n;
+    const double x1 = mc->time[mc->chunkCounter];
+    for (size_t k = 0; k < mc->n; ++k) out[k] = y0[k] + (t - x0) * (y1[k] - y0[k]) / (x1 - x0);
+}
+
 /* SWM_forcing_update, swm_forcing_module.f90:135-170 (+ processWindstress 274-324,
  * processOscillation 326-374) */
 static void forcing_update(orc_model *m) {
@@ -565,6 +654,28 @@ static void forcing_update(orc_model *m) {
         const double *R = m->d.osc_re[s], *I = m->d.osc_im[s];
         for (size_t k = 0; k < n; ++k)
             F[k] = F[k] + (2.0 * (R[k] * r_iot - I[k] * i_iot)) * (double)mask[k];
+    }
+    /* processWindstress 274-324, processCustomForcing 395-434, processHeating 535-552
+     * on getChunkData(memChunk, itt*dt) */
+    for (int s = 0; s < m->d.n_tdf; ++s) {
+        int comp = m->d.tdf_comp[s];
+        double *F = comp == ORC_COMP_X ? m->F_x : comp == ORC_COMP_Y ? m->F_y : m->F_eta;
+        const int8_t *mask = comp == ORC_COMP_X ? m->ocean[ORC_GRID_U]
+                           : comp == ORC_COMP_Y ? m->ocean[ORC_GRID_V] : m->ocean[ORC_GRID_ETA];
+        double *data = m->tdf_buf;
+        mc_get(&m->mc[s], (double)m->itt * m->d.dt, data);
+        if (m->d.tdf_kind[s] == ORC_TDF_WIND) {
+            const double *H = comp == ORC_COMP_X ? m->Du : m->Dv;
+            int scaleH = m->d.barotropic || m->d.variant == ORC_FULLY_NONLINEAR;
+            for (size_t k = 0; k < n; ++k) {
+                if (mask[k] != 1) continue;
+                double num = m->d.has_tau_scale ? m->d.tau_scale * data[k] : data[k];
+                F[k] = F[k] + num / (scaleH ? m->d.RHO0 * H[k] : m->d.RHO0);
+            }
+        } else {
+            for (size_t k = 0; k < n; ++k)
+                if (mask[k] == 1) F[k] = F[k] + data[k];
+        }
     }
 }
 
@@ -729,6 +840,9 @@ orc_model *orc_create(const orc_desc *d) {
     if (d->F_x_const) memcpy(m->F_x_const, d->F_x_const, n * sizeof(double));
     if (d->F_y_const) memcpy(m->F_y_const, d->F_y_const, n * sizeof(double));
     if (d->F_eta_const) memcpy(m->F_eta_const, d->F_eta_const, n * sizeof(double));
+    for (int s = 0; s < d->n_tdf; ++s)   /* SWM_forcing_initStream -> initMemChunk, 593-607 */
+        mc_init(&m->mc[s], d->tdf_rec[s], n, d->tdf_nrec[s], d->tdf_chunk[s], d->tdf_dt[s]);
+    m->tdf_buf = zalloc(n);
     forcing_update(m);
     /* SWM_initialConditions, swm_module.f90:156-181 */
     for (size_t k = 0; k < n; ++k) {
@@ -957,5 +1071,7 @@ void orc_destroy(orc_model *m) {
         free(tr->uhc); free(tr->vhc); free(tr->diff); free(tr->forcing);
     }
     free(m->TRC_coeff); free(m->sor_c); free(m->i_oe);
+    for (int s = 0; s < m->d.n_tdf; ++s) { free(m->mc[s].var); free(m->mc[s].time); }
+    free(m->tdf_buf);
     free(m);
 }

@@ -20,6 +20,7 @@
 enum { ORC_GRID_H = 0, ORC_GRID_U = 1, ORC_GRID_V = 2, ORC_GRID_ETA = 3 };
 enum { ORC_FULLY_NONLINEAR = 1, ORC_LINEARISED_MEAN_STATE = 2, ORC_LINEARISED_STATE_OF_REST = 3 };
 enum { ORC_COMP_X = 0, ORC_COMP_Y = 1, ORC_COMP_ETA = 2 };
+enum { ORC_TDF_WIND = 0, ORC_TDF_CUSTOM = 1, ORC_TDF_HEATING = 2 };
 
 typedef struct orc_grid {
     const double *H;        /* (Nx,Ny) layer thickness on this grid, as left by initDomain/computeD */
@@ -58,6 +59,15 @@ typedef struct orc_desc {
     double kappa_h[ORC_MAX_TRACERS];
     const double *trc_C[ORC_MAX_TRACERS], *trc_C0[ORC_MAX_TRACERS];
     const double *trc_gamma_C[ORC_MAX_TRACERS], *trc_cons[ORC_MAX_TRACERS];
+    /* time-interpolated forcing streams on memchunk data (memchunk_module.f90),
+     * applied after the wind and oscillating streams in the order given */
+    int32_t n_tdf;
+    int32_t tdf_kind[ORC_MAX_STREAMS];      /* ORC_TDF_WIND, ORC_TDF_CUSTOM, ORC_TDF_HEATING */
+    int32_t tdf_comp[ORC_MAX_STREAMS];      /* ORC_COMP_* */
+    int32_t tdf_nrec[ORC_MAX_STREAMS];      /* records in the dataset */
+    int32_t tdf_chunk[ORC_MAX_STREAMS];     /* namelist chunksize */
+    double tdf_dt[ORC_MAX_STREAMS];         /* record spacing in model seconds (getDt) */
+    const double *tdf_rec[ORC_MAX_STREAMS]; /* nrec (Nx,Ny) records back to back */
 } orc_desc;
 
 typedef struct orc_model orc_model;

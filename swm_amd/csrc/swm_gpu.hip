@@ -63,6 +63,106 @@ struct FieldDev {            // device pointers of an uploaded static field
     void* rows = nullptr;
 };
 
+// Index mirror of a memoryChunk (src/memchunk_module.f90:37-273).  The chunk's
+// slots hold record numbers instead of data (every record stays resident on
+// the device) and the time axis is built with the same arithmetic, so each
+// query returns the reference's bracketing records and abscissas.
+struct ChunkIdx {
+    int64_t nrec = 0, chunkSize = 0, fileRec = 1, chunkCounter = 1, nFpC = 0;
+    int64_t req_chunk = 0;
+    bool isInitialised = false, isPersistent = false, isConstant = false;
+    double dt = 0.0, tOffset = 0.0, req_dt = 0.0, last_t = -HUGE_VAL;
+    std::vector<int64_t> slot;   // 1-based record held by each slot
+    std::vector<double> time;
+
+    // getVarChunkFromDisk, 181-207 (nstart is updated as the reference's fileRec is)
+    void var_chunk(size_t at, int64_t& nstart, int64_t len) {
+        if (nstart > nrec) nstart = nstart - nFpC * nrec;
+        if (nstart == 0) nstart = nrec;
+        const int64_t nend = std::min(nstart + len - 1, nrec);
+        const int64_t len2 = std::max<int64_t>(nend - nstart, 0) + 1;
+        for (int64_t q = 0; q < len2; ++q) slot[at + q] = nstart + q;
+        if (len2 < len) {
+            int64_t nstart2 = 1;
+            var_chunk(at + len2, nstart2, len - len2);
+        }
+    }
+    // getChunkFromDisk, 149-170
+    void get_chunk() {
+        if (isInitialised) {
+            fileRec = fileRec + chunkSize - 1;
+            tOffset = tOffset - dt;
+        }
+        if (!isPersistent || !isInitialised) var_chunk(0, fileRec, chunkSize);
+        if (!isConstant)
+            for (int64_t i = 0; i < chunkSize; ++i) {
+                time[i] = tOffset;
+                tOffset = tOffset + dt;
+            }
+        chunkCounter = 1;
+    }
+    // initMemChunk, 77-115
+    void init(int64_t n, int64_t cs, double rec_dt) {
+        *this = ChunkIdx();
+        nrec = n;
+        req_chunk = cs;
+        req_dt = rec_dt;
+        chunkSize = cs;
+        if (n <= 1) {
+            isPersistent = isConstant = true;
+            chunkSize = 1;
+        } else if (cs >= n + 1) {
+            isPersistent = true;
+            chunkSize = n + 1;
+        }
+        dt = isConstant ? 0.0 : rec_dt;
+        slot.assign((size_t)chunkSize, 1);
+        time.assign((size_t)chunkSize, 0.0);
+        get_chunk();
+        nFpC = std::max<int64_t>((int64_t)std::floor((float)chunkSize / (float)nrec), 1);
+        isInitialised = true;
+    }
+    // getChunkData, 214-243: records r0, r1 (0-based) and the interpolation
+    // scalars a = t - time(cc), b = time(cc+1) - time(cc).  Model time only
+    // moves forward in the reference; an earlier time (a context resumed or
+    // restored at a smaller itt) starts from a fresh chunk.
+    void query(double t, int64_t& r0, int64_t& r1, bool& exact, double& a, double& b) {
+        if (t < last_t) init(nrec, req_chunk, req_dt);
+        last_t = t;
+        if (isConstant) {
+            r0 = r1 = slot[0] - 1;
+            exact = true;
+            a = 0.0;
+            b = 1.0;
+            return;
+        }
+        if (chunkCounter >= chunkSize) get_chunk();
+        while (time[(size_t)chunkCounter] < t) {
+            chunkCounter = chunkCounter + 1;
+            if (chunkCounter >= chunkSize) get_chunk();
+        }
+        const double x0 = time[(size_t)chunkCounter - 1];
+        r0 = slot[(size_t)chunkCounter - 1] - 1;
+        if (t == x0) {
+            r1 = r0;
+            exact = true;
+            a = 0.0;
+            b = 1.0;
+            return;
+        }
+        r1 = slot[(size_t)chunkCounter] - 1;
+        exact = false;
+        a = t - x0;
+        b = time[(size_t)chunkCounter] - x0;
+    }
+};
+
+struct FStream {
+    int kind = 0, comp = 0;      // SWM_STREAM_*, SWM_COMP_*
+    void* rec = nullptr;         // nrec pitched planes of the context's type
+    ChunkIdx idx;
+};
+
 struct Tracer {
     double kappa = 0;
     bool cbuf_valid = false;
@@ -95,6 +195,7 @@ struct swm_ctx {
     void* statics = nullptr;     // device Statics<T>
     double ab[5][5] = {};
     std::vector<Tracer> trc;
+    std::vector<FStream> streams;   // time-interpolated forcing streams, in processing order
     int tile = 0;
     bool use_graph = true;
     int num_cu = 0;              // compute units of the device
@@ -388,6 +489,13 @@ int write_statics_t(swm_ctx* c) {
         for (int k = 0; k < 3; ++k) a.tdiag[n][k] = (T*)c->tdiag[n][k];
     }
     for (int k = 0; k < 14; ++k) a.diag[k] = (T*)c->diag[k];
+    a.nstream = (int)c->streams.size();
+    a.sdiv = (a.barotropic || d.variant == SWM_FULLY_NONLINEAR) ? 1 : 0;
+    for (size_t n = 0; n < c->streams.size(); ++n) {
+        a.skind[n] = c->streams[n].kind == SWM_STREAM_WIND ? 0 : 1;
+        a.scomp[n] = c->streams[n].comp;
+        a.srec[n] = (const T*)c->streams[n].rec;
+    }
     if (!c->statics && !(c->statics = dalloc(c, sizeof(Statics<T>)))) return fail(c, "out of device memory", 3);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(c->statics, &a, sizeof(a), hipMemcpyHostToDevice));
@@ -563,6 +671,23 @@ void set_osc(swm_ctx* c, Dyn<T>& d, double tf) {
     }
 }
 
+// Place of every time-interpolated stream at model time tf*dt (tf = itt of
+// the SWM_forcing_update that set the forcing; the reference passes itt*dt).
+template <typename T>
+void set_streams(swm_ctx* c, Dyn<T>& d, double tf) {
+    for (size_t n = 0; n < c->streams.size(); ++n) {
+        int64_t r0 = 0, r1 = 0;
+        bool exact = true;
+        double a = 0.0, b = 1.0;
+        c->streams[n].idx.query(tf * c->d.dt, r0, r1, exact, a, b);
+        d.ss[n].o0 = r0 * c->plane;
+        d.ss[n].o1 = r1 * c->plane;
+        d.ss[n].exact = exact ? 1 : 0;
+        d.ss[n].a = (T)a;
+        d.ss[n].b = (T)b;
+    }
+}
+
 template <typename T>
 int step_one(swm_ctx* c) {
     c->itt += 1;
@@ -574,6 +699,7 @@ int step_one(swm_ctx* c) {
     d.dtc3 = (T)(c->d.dt * c->ab[tstep - 1][2]);
     // the forcing used by step itt was set by SWM_forcing_update at itt-1
     set_osc(c, d, (double)(c->itt - 1));
+    set_streams(c, d, (double)(c->itt - 1));
     int rc = launch_step_t<T, false>(c, d, c->tile);
     if (rc) return rc;
     // model_advance: N0 <- N0p1, G(1) <- G(2), G(2) <- G(3)
@@ -597,7 +723,8 @@ void drop_graphs(swm_ctx* c) {
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
 }
 
-// Steady state (itt >= 3, no first-step tracer buffer, no oscillating stream):
+// Steady state (itt >= 3, no first-step tracer buffer, no oscillating or
+// time-interpolated stream, whose per-step arguments change every step):
 // the launch arguments repeat with the period 6 of the ping-pong x ring roles,
 // so six steps are captured once per phase and replayed as one graph.
 template <typename T>
@@ -606,7 +733,7 @@ int step_many(swm_ctx* c, int64_t nsteps) {
     while (done < nsteps) {
         bool cbuf = false;
         for (auto& tr : c->trc) cbuf = cbuf || tr.cbuf_valid;
-        const bool steady = c->use_graph && !c->has_osc && !cbuf && c->itt >= 2;
+        const bool steady = c->use_graph && !c->has_osc && c->streams.empty() && !cbuf && c->itt >= 2;
         if (steady && nsteps - done >= 6) {
             const int key = c->cur * 3 + c->ring;
             if (!c->graphs[key]) {
@@ -661,6 +788,7 @@ int materialise(swm_ctx* c) {
     Dyn<T> d = make_dyn<T>(c, 2);   // the materialisation kernel uses the 64 x 8 tile (kTiles[2])
     // forcing as left by the last SWM_forcing_update
     set_osc(c, d, (double)c->itt);
+    set_streams(c, d, (double)c->itt);
     return launch_step_t<T, true>(c, d, 0);
 }
 
@@ -1193,6 +1321,41 @@ int swm_gpu_set_kernel_shape(swm_ctx* c, int32_t shape) {
 
 int swm_gpu_kernel_shape_count(void) { return kNumTiles; }
 
+int swm_gpu_add_forcing_stream(swm_ctx* c, int32_t kind, int32_t component, int32_t nrec, double rec_dt,
+                               int32_t chunk_size, const double* records, int32_t* stream_id) {
+    if (!c || !records || !stream_id) return 1;
+    HipLock lock(g_hip_mu);
+    if (int rc = member_refresh(c)) return rc;
+    if (c->batch) return fail(c, "context is attached to a batch; add streams before attaching", 1);
+    if (kind != SWM_STREAM_WIND && kind != SWM_STREAM_CUSTOM && kind != SWM_STREAM_HEATING)
+        return fail(c, "unknown forcing stream kind", 1);
+    if (component < SWM_COMP_X || component > SWM_COMP_ETA) return fail(c, "unknown component", 1);
+    if (kind == SWM_STREAM_WIND && component == SWM_COMP_ETA)
+        return fail(c, "windstress acts on the momentum equations (SWM_COMP_X, SWM_COMP_Y)", 1);
+    if (kind == SWM_STREAM_HEATING && component != SWM_COMP_ETA)
+        return fail(c, "heating acts on the continuity equation (SWM_COMP_ETA)", 1);
+    if (nrec < 1) return fail(c, "a stream needs at least one record", 1);
+    if (nrec > 1 && chunk_size < 2) return fail(c, "a time-dependent dataset needs chunk_size >= 2", 1);
+    if (nrec > 1 && !(rec_dt > 0.0)) return fail(c, "record spacing must be positive", 1);
+    if (c->streams.size() >= 4) return fail(c, "at most 4 time-interpolated forcing streams", 1);
+    FStream st;
+    st.kind = kind;
+    st.comp = component;
+    st.rec = dalloc(c, (size_t)nrec * c->plane * c->esz);
+    if (!st.rec) return fail(c, "out of device memory for the stream records", 3);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemset(st.rec, 0, (size_t)nrec * c->plane * c->esz));
+    for (int32_t r = 0; r < nrec; ++r) {
+        int rc = upload_grid(c, (char*)st.rec + (size_t)r * c->plane * c->esz, records + (size_t)r * c->Nx * c->Ny);
+        if (rc) return rc;
+    }
+    st.idx.init(nrec, chunk_size, rec_dt);
+    c->streams.push_back(std::move(st));
+    drop_graphs(c);
+    *stream_id = (int32_t)c->streams.size() - 1;
+    return write_statics(c);
+}
+
 int swm_gpu_tune(swm_ctx* c, int32_t steps, int32_t* best_shape, double* times_ms) {
     if (!c || !best_shape) return 1;
     if (c->batch) return fail(c, "context is attached to a batch; tune the batch instead", 1);
@@ -1582,6 +1745,8 @@ int swm_gpu_batch_attach(swm_batch* b, swm_ctx* c, int64_t nsteps) {
     if (c->batch) return bfail(b, "context is already attached to a batch", 1);
     if (nsteps < 0) return bfail(b, "nsteps must be >= 0", 1);
     if (c->has_osc) return bfail(b, "oscillating forcing needs per-step host phases; step such a context alone", 4);
+    if (!c->streams.empty())
+        return bfail(b, "time-interpolated forcing streams need per-step host arguments; step such a context alone", 4);
     if (!c->avgs.empty()) return bfail(b, "averaging buffers are kept by contexts stepped alone", 4);
     if (b->fp32 >= 0 && (b->fp32 != (int)c->fp32 || b->variant != c->d.variant))
         return bfail(b, "every member of a batch has the same precision and variant", 4);

@@ -62,6 +62,11 @@ struct Statics {
     // tracers
     T kappa[2];
     Fld<T> C0[2], gC[2], timpl[2];
+    // time-interpolated forcing streams: kind 0 = windstress, 1 = custom or heating
+    // (added as is); component 0 = x (ocean u), 1 = y (ocean v), 2 = eta
+    int nstream, sdiv;   // sdiv: windstress divides by RHO0 * D_face (BAROTROPIC or FULLY_NONLINEAR)
+    int skind[4], scomp[4];
+    const T* srec[4];    // records of each stream, one plane apart
     // diagnostic outputs (materialisation kernel only)
     T* diag[14];      // D Dh Du Dv zeta Pot EDens MU MV F_x F_y F_eta latmix_u latmix_v
     T* tdiag[2][3];   // C uhc vhc per tracer
@@ -70,6 +75,16 @@ struct Statics {
 // State planes in one allocation: u, v, eta at two levels, three AB slots per
 // field, then per tracer CH (two levels), G_CH (three slots) and the C buffer.
 enum : int { PL_U = 0, PL_V = 2, PL_E = 4, PL_GU = 6, PL_GV = 9, PL_GE = 12, PL_TRC = 15, PL_PER_TRC = 6 };
+
+// One forcing stream's place on its time axis for a step: records o0 and o1
+// (element offsets) bracket the model time; the value is rec[o0] when exact,
+// else rec[o0] + a*(rec[o1] - rec[o0])/b with a = t - t0, b = t1 - t0.
+template <typename T>
+struct SStep {
+    long long o0, o1;
+    int exact;
+    T a, b;
+};
 
 // Per-launch arguments (by value).
 template <typename T>
@@ -85,6 +100,7 @@ struct Dyn {
     int cbuf;            // bit n: tracer n reads its C buffer (first step after hand-over)
     T dtc1, dtc2, dtc3;  // dt*ab_coeff(ti, tstep)
     T oc[3], os[3];      // cos/sin(omega*itt*dt) of the oscillating streams (x, y, eta)
+    SStep<T> ss[4];      // time-interpolated forcing streams
 };
 
 // weight = 1 / max(sum of masks, 1) of the *_noland interpolators
@@ -203,6 +219,33 @@ __device__ __forceinline__ void gst(T* p, size_t i, T v) {
 }
 template <typename T>
 using CStatics = const __attribute__((address_space(4))) Statics<T>;
+
+// Time-interpolated forcing streams, after the descriptor's windstress and
+// oscillating streams: SWM_forcing_processWindstress / processCustomForcing /
+// processHeating (src/swm_forcing_module.f90:274-324, 395-434, 535-552) on
+// getChunkData (src/memchunk_module.f90:214-243), which interpolates between
+// the bracketing records as interp1D_array2D, y0 + (t-t0)*(y1-y0)/(t1-t0)
+// (src/calc_lib.f90:386-397).  off is the cell's offset within a plane.
+template <typename T, typename S>
+__device__ __forceinline__ void apply_streams(S& a, const Dyn<T>& d, size_t off, int ou, int ov, int oe, T Du, T Dv,
+                                              T& Fx, T& Fy, T& Fe) {
+    const int ns = a.nstream;
+    for (int n = 0; n < ns; ++n) {
+        const int comp = a.scomp[n];
+        if (!(comp == 0 ? ou : comp == 1 ? ov : oe)) continue;
+        const T* r = a.srec[n];
+        const T y0 = gld(r + d.ss[n].o0, off);
+        T y = y0;
+        if (!d.ss[n].exact) y = y0 + dv(d.ss[n].a * (gld(r + d.ss[n].o1, off) - y0), d.ss[n].b);
+        if (a.skind[n] == 0) {
+            const T num = a.has_tau_scale ? a.tau_scale * y : y;
+            y = a.sdiv ? dv(num, a.RHO0 * (comp == 0 ? Du : Dv)) : dv(num, a.RHO0);
+        }
+        if (comp == 0) Fx = Fx + y;
+        else if (comp == 1) Fy = Fy + y;
+        else Fe = Fe + y;
+    }
+}
 
 // Registers of one tile between its loads and their LDS stores: the state with
 // halo (NE cells per thread), the per-row tables and the AB history of the
@@ -619,6 +662,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             Fy = Fy + dv(a.has_tau_scale ? a.tau_scale * ty : ty, a.RHO0 * Dv);
         }
         if (a.has_oy) Fy = Fy + (T(2) * (fl(a.oRy, off, ov) * d.oc[1] - fl(a.oIy, off, ov) * d.os[1])) * T(ov);
+        if (a.nstream) apply_streams<T>(a, d, off, ou, ov, oe, Du, Dv, Fx, Fy, Fe);
 
         // SWM_timestep_nonlinear (src/swm_timestep_module.f90:349-443)
         T Ge = T(0), Gu = T(0), Gv = T(0), Lu = T(0), Lv = T(0);
@@ -1121,6 +1165,7 @@ march_kernel(const Dyn<T> d) {
                         Fy = Fy + dv(a.has_tau_scale ? a.tau_scale * ty : ty, a.RHO0 * Dv);
                     }
                     if (a.has_oy) Fy = Fy + (T(2) * (fld_r(a.oRy, ro, gi, j, ov) * d.oc[1] - fld_r(a.oIy, ro, gi, j, ov) * d.os[1])) * T(ov);
+                    if (a.nstream) apply_streams<T>(a, d, ro + gi, ou, ov, oe, Du, Dv, Fx, Fy, Fe);
 
                     // SWM_timestep_nonlinear
                     T Ge = T(0), Gu = T(0), Gv = T(0);

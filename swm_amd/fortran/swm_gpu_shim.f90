@@ -16,6 +16,8 @@ module swm_gpu_shim
   integer(c_int32_t), parameter, public :: SWM_LATERAL_MIXING = 1, SWM_QUADRATIC_BOTTOM_FRICTION = 2, &
                                            SWM_BAROTROPIC = 4, SWM_TAU_SCALE = 8
   integer(c_int32_t), parameter, public :: SWM_FP64 = 0, SWM_FP32 = 1
+  integer(c_int32_t), parameter, public :: SWM_COMP_X = 0, SWM_COMP_Y = 1, SWM_COMP_ETA = 2
+  integer(c_int32_t), parameter, public :: SWM_STREAM_WIND = 0, SWM_STREAM_CUSTOM = 1, SWM_STREAM_HEATING = 2
 
   !> swm_grid_desc
   type, bind(C), public :: swm_grid_desc
@@ -64,6 +66,16 @@ module swm_gpu_shim
       import :: c_int, c_ptr, c_int32_t, swm_tracer_desc
       type(c_ptr), value :: ctx
       type(swm_tracer_desc), intent(in) :: t
+      integer(c_int32_t), intent(out) :: id
+    end function
+    ! SWM_forcing_initStream for a dataset of nrec records (memchunk); records(Nx,Ny,nrec)
+    integer(c_int) function swm_gpu_add_forcing_stream(ctx, kind, component, nrec, rec_dt, chunk_size, &
+                                                        records, id) bind(C, name="swm_gpu_add_forcing_stream")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: kind, component, nrec, chunk_size
+      real(c_double), value :: rec_dt
+      real(c_double), intent(in) :: records(*)
       integer(c_int32_t), intent(out) :: id
     end function
     integer(c_int) function swm_gpu_step(ctx, nsteps) bind(C, name="swm_gpu_step")
@@ -136,7 +148,8 @@ module swm_gpu_shim
     end subroutine
   end interface
 
-  public :: swm_gpu_create, swm_gpu_upload_state, swm_gpu_add_tracer, swm_gpu_step, swm_gpu_itt, &
+  public :: swm_gpu_create, swm_gpu_upload_state, swm_gpu_add_tracer, swm_gpu_add_forcing_stream, &
+            swm_gpu_step, swm_gpu_itt, &
             swm_gpu_sync, swm_gpu_sor, swm_gpu_velocity_potential, swm_gpu_avg_add, swm_gpu_avg_read, &
             swm_gpu_sums, swm_gpu_tune, swm_gpu_last_error, swm_gpu_destroy
   public :: gpu_check, gpu_sync_field, gpu_main_ElSolv_SOR, gpu_avg_register, gpu_avg_write

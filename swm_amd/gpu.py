@@ -36,7 +36,7 @@ EXPORTED = [
     "swm_gpu_batch_remaining", "swm_gpu_batch_step", "swm_gpu_batch_last_timing", "swm_gpu_batch_last_error",
     "swm_gpu_batch_destroy", "swm_gpu_kernel_shape_count", "swm_gpu_set_kernel_shape", "swm_gpu_tune",
     "swm_gpu_set_sor_check_every", "swm_gpu_batch_set_shape", "swm_gpu_batch_tune", "swm_gpu_avg_add",
-    "swm_gpu_avg_read",
+    "swm_gpu_avg_read", "swm_gpu_add_forcing_stream",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -88,6 +88,9 @@ def load_library(path: str = LIB_PATH):
     L.swm_gpu_upload_state.restype = C.c_int
     L.swm_gpu_add_tracer.argtypes = [p, C.POINTER(TracerDesc), C.POINTER(C.c_int32)]
     L.swm_gpu_add_tracer.restype = C.c_int
+    L.swm_gpu_add_forcing_stream.argtypes = [p, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, _dp,
+                                             C.POINTER(C.c_int32)]
+    L.swm_gpu_add_forcing_stream.restype = C.c_int
     L.swm_gpu_step.argtypes = [p, C.c_int64]
     L.swm_gpu_step.restype = C.c_int
     L.swm_gpu_itt.argtypes = [p]
@@ -235,6 +238,8 @@ class SwmGpu:
             raise SwmGpuError(f"swm_gpu_create failed ({rc}): {self.L.swm_gpu_last_error(None).decode()}")
         self._h = h
         self._keep = []   # inputs are copied to the device during create
+        for st in getattr(s, "streams", []):
+            self.add_forcing_stream(st.kind, st.comp, st.records, st.rec_dt, st.chunk_size)
 
     # -- error handling -------------------------------------------------------
     def _check(self, rc: int, what: str):
@@ -272,6 +277,20 @@ class SwmGpu:
         tid = C.c_int32()
         self._check(self.L.swm_gpu_add_tracer(self._h, C.byref(t), C.byref(tid)), "swm_gpu_add_tracer")
         return tid.value
+
+    def add_forcing_stream(self, kind: int, comp: int, records, rec_dt: float, chunk_size: int = 100) -> int:
+        """A time-interpolated forcing stream (swm_gpu_add_forcing_stream):
+        records (nrec, Ny, Nx) spaced rec_dt model seconds."""
+        rec = np.ascontiguousarray(records, dtype=np.float64)
+        if rec.ndim == 2:
+            rec = rec[None]
+        if rec.shape[1:] != (self.s.Ny, self.s.Nx):
+            raise SwmGpuError("records must be (nrec, Ny, Nx)")
+        sid = C.c_int32()
+        self._check(self.L.swm_gpu_add_forcing_stream(self._h, int(kind), int(comp), rec.shape[0], float(rec_dt),
+                                                      int(chunk_size), rec.ctypes.data_as(_dp), C.byref(sid)),
+                    "swm_gpu_add_forcing_stream")
+        return sid.value
 
     def init_from_setup(self):
         """SWM_initialConditions + TRC_initTracer hand-over for a fresh run."""

@@ -203,6 +203,22 @@ def noland_interp(D: np.ndarray, ocean_eta: np.ndarray, target: str) -> np.ndarr
     return out
 
 
+STREAM_WIND, STREAM_CUSTOM, STREAM_HEATING = 0, 1, 2
+DEF_FORCING_CHUNKSIZE = 100     # SWM_DEF_FORCING_CHUNKSIZE, include/swm_module.h:30
+
+
+@dataclass
+class ForcingStream:
+    """A time-dependent forcing stream read through a memoryChunk
+    (src/memchunk_module.f90): nrec (Ny, Nx) records spaced rec_dt model
+    seconds (getDt), record 1 at model time 0, cycling after the last."""
+    kind: int                  # STREAM_WIND ("W"), STREAM_CUSTOM ("C"), STREAM_HEATING ("H")
+    comp: int                  # COMP_X, COMP_Y, COMP_ETA
+    records: np.ndarray        # (nrec, Ny, Nx)
+    rec_dt: float
+    chunk_size: int = DEF_FORCING_CHUNKSIZE
+
+
 @dataclass
 class Setup:
     """Everything the hot path reads, as the reference holds it after init."""
@@ -238,6 +254,7 @@ class Setup:
     F_eta_const: np.ndarray = None
     wind: list = field(default_factory=list)        # [(comp, tau)] re-applied every step
     osc: list = field(default_factory=list)         # [(comp, re, im, omega)]
+    streams: list = field(default_factory=list)     # [ForcingStream] time-interpolated, after wind and osc
     u_bs: np.ndarray = None
     v_bs: np.ndarray = None
     zeta_bs: np.ndarray = None
@@ -354,21 +371,24 @@ def init_damping(s: Setup, terms: list, loader=None) -> None:
 # forcing, swm_forcing_module.f90
 # ---------------------------------------------------------------------------
 
-def add_windstress(s: Setup, comp: int, tau: np.ndarray, constant_in_time: bool = True) -> None:
-    """A windstress stream (processWindstress, swm_forcing_module.f90:274-324).
+def add_windstress(s: Setup, comp: int, tau: np.ndarray) -> None:
+    """A windstress stream of one record (processWindstress,
+    swm_forcing_module.f90:274-324).
 
     FULLY_NONLINEAR makes windstress time dependent (initStream 631-632): it is
     re-applied every step with the current Du/Dv, so it is kept as a stream.
     Otherwise a constant stream is folded once into F_*_const with the
     (overwritten) H_u/H_v, as SWM_forcing_getForcing(isTDF=.FALSE.) does.
+    Datasets of several records go through add_forcing_stream.
     """
     for name in ("F_x_const", "F_y_const", "F_eta_const"):
         if getattr(s, name) is None:
             setattr(s, name, np.zeros(s.shape))
     tau = np.asarray(tau, dtype=np.float64)
-    if s.variant == FULLY_NONLINEAR or not constant_in_time:
-        if not constant_in_time and s.variant != FULLY_NONLINEAR:
-            raise NotImplementedError("time-interpolated windstress streams (memchunk) are not supported")
+    if s.variant == FULLY_NONLINEAR:
+        if s.osc or s.streams:
+            raise NotImplementedError("windstress streams are applied before oscillating and time-interpolated "
+                                      "streams; list them first")
         s.wind.append((comp, tau))
         return
     F = s.F_x_const if comp == COMP_X else s.F_y_const
@@ -384,7 +404,46 @@ def add_oscillation(s: Setup, comp: int, re: np.ndarray, im: np.ndarray, omega: 
     for name in ("F_x_const", "F_y_const", "F_eta_const"):
         if getattr(s, name) is None:
             setattr(s, name, np.zeros(s.shape))
+    if s.streams:
+        raise NotImplementedError("oscillating streams are applied before time-interpolated streams; list them first")
     s.osc.append((comp, np.asarray(re, dtype=np.float64), np.asarray(im, dtype=np.float64), float(omega)))
+
+
+def add_forcing_stream(s: Setup, kind: int, comp: int, records: np.ndarray, rec_dt: float,
+                       chunk_size: int = DEF_FORCING_CHUNKSIZE) -> None:
+    """A forcing stream on a dataset of several records, interpolated in time
+    every step (SWM_forcing_initStream, swm_forcing_module.f90:593-640;
+    getChunkData, memchunk_module.f90:214-243).  A dataset of one record is
+    constant in time and is folded into F_*_const as the reference does
+    (except windstress under FULLY_NONLINEAR, which add_windstress keeps as a
+    per-step stream).  Time-dependent streams are processed in the order they
+    are added, after the windstress and oscillating streams."""
+    for name in ("F_x_const", "F_y_const", "F_eta_const"):
+        if getattr(s, name) is None:
+            setattr(s, name, np.zeros(s.shape))
+    rec = np.asarray(records, dtype=np.float64)
+    if rec.ndim == 2:
+        rec = rec[None]
+    if rec.shape[1:] != s.shape:
+        raise ValueError(f"records must be (nrec, {s.Ny}, {s.Nx})")
+    if kind == STREAM_WIND and comp == COMP_ETA:
+        raise ValueError("windstress has components X (zonal) and Y (meridional) only")
+    if kind == STREAM_HEATING and comp != COMP_ETA:
+        raise ValueError("heating acts on the continuity equation (COMP_ETA)")
+    if rec.shape[0] <= 1:
+        if kind == STREAM_WIND:
+            add_windstress(s, comp, rec[0])
+            return
+        F = {COMP_X: s.F_x_const, COMP_Y: s.F_y_const, COMP_ETA: s.F_eta_const}[comp]
+        g = s.grids[{COMP_X: "u", COMP_Y: "v", COMP_ETA: "eta"}[comp]]
+        F[:] = np.where(g.ocean == 1, F + rec[0], F)
+        return
+    if chunk_size < 2:
+        raise ValueError("a time-dependent dataset needs chunk_size >= 2 (getChunkData reads time(chunkCounter+1))")
+    if len(s.streams) >= 4:
+        raise ValueError("at most 4 time-interpolated forcing streams")
+    s.streams.append(ForcingStream(kind=kind, comp=comp, records=np.ascontiguousarray(rec), rec_dt=float(rec_dt),
+                                   chunk_size=int(chunk_size)))
 
 
 def finalize(s: Setup) -> Setup:
@@ -431,6 +490,31 @@ def _read_nc(path: str, varname: str):
                 mv = np.asarray(getattr(v, att)).astype(np.float64)
                 miss[data.astype(np.float64) == mv] = 1
         return data.astype(np.float64), miss
+
+
+_SECONDS_PER = {"second": 1.0, "seconds": 1.0, "s": 1.0, "minute": 60.0, "minutes": 60.0, "hour": 3600.0,
+                "hours": 3600.0, "h": 3600.0, "day": 86400.0, "days": 86400.0, "d": 86400.0}
+
+
+def _read_records(path: str, varname: str):
+    """Every record of a forcing dataset and its record spacing in model
+    seconds, dt = (t_last - t_first)/(nrec - 1) (getDt,
+    memchunk_module.f90:252-273; the calendar offset cancels in the
+    difference, only the unit's length matters)."""
+    from scipy.io import netcdf_file
+    with netcdf_file(path, "r", mmap=False) as nc:
+        v = nc.variables[varname]
+        data = np.array(v.data, dtype=np.float64)
+        if data.ndim == 2:
+            return data[None], 0.0
+        if data.shape[0] <= 1:
+            return data, 0.0
+        tname = v.dimensions[0]
+        t = np.array(nc.variables[tname].data, dtype=np.float64)
+        units = nc.variables[tname].units
+        units = units.decode() if isinstance(units, bytes) else str(units)
+        scale = _SECONDS_PER[units.split()[0].lower()]
+    return data, (t[-1] * scale - t[0] * scale) / float(np.float32(data.shape[0] - 1.0))
 
 
 def setup_from_selfcheck(exp_dir: str, defines: dict) -> Setup:
@@ -485,12 +569,12 @@ def setup_from_selfcheck(exp_dir: str, defines: dict) -> Setup:
             re, _ = _read_nc(path(st["filename"]), st["varname"])
             im, _ = _read_nc(path(st["filename2"]), st["varname2"])
             add_oscillation(s, comp, re, im, float(st.get("omega", 0.0)))
-        elif ftype == "W":
-            from scipy.io import netcdf_file
-            with netcdf_file(path(st["filename"]), "r", mmap=False) as nc:
-                nrec = nc.variables[st["varname"]].data.shape[0] if nc.variables[st["varname"]].data.ndim == 3 else 1
-            tau, _ = _read_nc(path(st["filename"]), st["varname"])
-            add_windstress(s, comp, tau, constant_in_time=(nrec <= 1))
+        elif ftype in ("W", "C", "H"):
+            rec, rec_dt = _read_records(path(st["filename"]), st["varname"])
+            kind = {"W": STREAM_WIND, "C": STREAM_CUSTOM, "H": STREAM_HEATING}[ftype]
+            if ftype == "H":
+                comp = COMP_ETA
+            add_forcing_stream(s, kind, comp, rec, rec_dt, int(st.get("chunksize", DEF_FORCING_CHUNKSIZE)))
         else:
             raise NotImplementedError(f"forcing type {ftype}")
     if model_nl.get("file_eta_init"):
