@@ -146,7 +146,10 @@ int64_t swm_gpu_itt(const swm_ctx *c);
  * SWM_D SWM_DH SWM_DU SWM_DV SWM_EDENS SWM_POT SWM_RELVORT SWM_MU SWM_MV
  * F_X F_Y F_ETA SWM_LATMIX_U SWM_LATMIX_V, and for tracer_id >= 0:
  * CH C G_CH uhc vhc.  Derived fields are materialised on demand from the
- * device state.  Blocks. */
+ * device state.  PSI is the diagnostic streamfunction of u, v at N0
+ * (diagVar DVARNAME_PSI -> computeStreamfunction, src/diagVar.f90:122-123,
+ * src/calc_lib.f90:727-780, as built without an elliptic solver), computed on
+ * the device in FP64.  Blocks. */
 int swm_gpu_sync(swm_ctx *c, const char *name, int32_t tracer_id, double *host_out);
 
 /* main_ElSolv_SOR(B, chi, epsilon, first_guess) contract

@@ -82,6 +82,7 @@ def lib():
                                     C.POINTER(C.c_int64), _dp]
         L.orc_sor_fixed.restype = C.c_int
         L.orc_divergence.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.orc_psi.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_sor_apply.argtypes = [C.c_void_p, _dp, _dp]
         _lib = L
     return _lib
@@ -182,6 +183,14 @@ class Oracle:
         v = np.ascontiguousarray(v, np.float64)
         out = np.empty(self.s.shape)
         lib().orc_divergence(self._h, u.ctypes.data_as(_dp), v.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+        return out
+
+    def psi(self, u=None, v=None):
+        """computeStreamfunction of (u, v); default the master u, v at N0."""
+        u = self.get("u") if u is None else np.ascontiguousarray(u, np.float64)
+        v = self.get("v") if v is None else np.ascontiguousarray(v, np.float64)
+        out = np.empty(self.s.shape)
+        lib().orc_psi(self._h, u.ctypes.data_as(_dp), v.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
         return out
 
     def sor_apply(self, chi):

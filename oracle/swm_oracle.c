@@ -887,6 +887,47 @@ void orc_step(orc_model *m, int64_t nsteps) {
 
 int64_t orc_itt(const orc_model *m) { return m->itt; }
 
+/* computeStreamfunction, calc_lib.f90:727-780, without an elliptic solver
+ * (u_nd = u, calc_lib.f90:563-564): path integrals of the (BAROTROPIC:
+ * thickness-weighted) flow; every SUM runs from its first to its last element. */
+void orc_psi(const orc_model *m, const double *u, const double *v, double *psi) {
+    const int Nx = m->Nx, Ny = m->Ny;
+    const int bt = m->d.barotropic;
+    const double A = m->d.A, dL = m->d.dLambda, dT = m->d.dTheta;
+    const double *Hu = m->H[ORC_GRID_U], *Hv = m->H[ORC_GRID_V], *cv = m->cosl[ORC_GRID_V];
+    const int8_t *ou = m->ocean[ORC_GRID_U], *ov = m->ocean[ORC_GRID_V];
+    for (size_t k = 0; k < (size_t)Nx * Ny; ++k) psi[k] = 0.0;
+    for (int i = 0; i < Nx - 1; ++i) {          /* psi(i,1), i = 1..Nx-1 */
+        double s = 0.0;
+        for (int q = i; q < Nx - 1; ++q) {
+            size_t k = IX(m, q, 0);
+            s = s + (bt ? Hv[k] * (double)ov[k] * v[k] : (double)ov[k] * v[k]);
+        }
+        psi[IX(m, i, 0)] = -s * A * cv[0] * dL;
+    }
+    for (int j = 1; j < Ny; ++j) {              /* psi(Nx,j), j = 2..Ny */
+        double s = 0.0;
+        for (int q = 0; q < j; ++q) {
+            size_t k = IX(m, Nx - 1, q);
+            s = s + (bt ? Hu[k] * (double)ou[k] * u[k] : (double)ou[k] * u[k]);
+        }
+        psi[IX(m, Nx - 1, j)] = -s * A * dT;
+    }
+    for (int j = 1; j < Ny; ++j)
+        for (int i = 0; i < Nx - 1; ++i) {
+            double s1 = 0.0, s2 = 0.0;
+            for (int q = i; q < Nx - 1; ++q) {
+                size_t k = IX(m, q, j);
+                s1 = s1 + (bt ? Hv[k] * v[k] : v[k]);
+            }
+            for (int q = 0; q < j; ++q) {
+                size_t k = IX(m, i, q);
+                s2 = s2 + (bt ? Hu[k] * u[k] : u[k]);
+            }
+            psi[IX(m, i, j)] = (-s1 * A * cv[j] * dL + psi[IX(m, Nx - 1, j)] - s2 * A * dT + psi[IX(m, i, 0)]) / 2.0;
+        }
+}
+
 int orc_get(const orc_model *m, const char *name, int tracer, double *out) {
     size_t n = (size_t)m->Nx * m->Ny;
     const double *src = NULL;

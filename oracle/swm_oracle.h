@@ -87,6 +87,8 @@ int orc_sor_fixed(orc_model *m, const double *B, double *chi, double omega, doub
                   int64_t max_sweeps, int64_t *iters, double *rel_res);
 /* apply the SOR operator (L chi) on ocean eta points */
 void orc_sor_apply(orc_model *m, const double *chi, double *out);
+/* computeStreamfunction(u, v, psi) without an elliptic solver (calc_lib.f90:727-780) */
+void orc_psi(const orc_model *m, const double *u, const double *v, double *psi);
 /* computeDivergence(u, v, div, u_grid, v_grid) */
 void orc_divergence(const orc_model *m, const double *u, const double *v, double *div);
 #endif

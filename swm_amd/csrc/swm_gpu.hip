@@ -221,6 +221,7 @@ struct swm_ctx {
     void* diag[14] = {};
     void* tdiag[kMaxTracers][3] = {};
     void* scratch = nullptr;     // one plane (divergence output)
+    double* psi_buf[4] = {};     // FP64 planes: pu, pv, column sums, psi
     // SOR
     void* sor_tab[4] = {};
     void* sor_relax = nullptr;
@@ -1033,12 +1034,53 @@ int swm_gpu_last_timing(const swm_ctx* c, double* total_ms, double* step_kernel_
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// computeStreamfunction of the master u, v at N0 into psi_buf[3].
+template <typename T>
+int psi_t(swm_ctx* c) {
+    const size_t bytes = (size_t)c->plane * sizeof(double);
+    for (auto& b : c->psi_buf)
+        if (!b && !(b = (double*)dalloc(c, bytes))) return fail(c, "out of device memory", 3);
+    const Statics<T>* sp = (const Statics<T>*)c->statics;
+    const T* u = (const T*)plane_ptr(c, swm::PL_U + c->cur);
+    const T* v = (const T*)plane_ptr(c, swm::PL_V + c->cur);
+    const long n = (long)c->Nx * c->Ny;
+    hipLaunchKernelGGL(swm::psi_prep<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, sp, u, v,
+                       c->psi_buf[0], c->psi_buf[1]);
+    hipLaunchKernelGGL(swm::psi_col<T>, dim3((c->Nx + 255) / 256), dim3(256), 0, c->stream, sp,
+                       (const double*)c->psi_buf[0], c->psi_buf[2], c->psi_buf[3]);
+    const unsigned segs = (unsigned)((c->Nx - 1 + 255) / 256);
+    hipLaunchKernelGGL(swm::psi_row<T>, dim3(segs, 1), dim3(256), 0, c->stream, sp, 0,
+                       (const double*)c->psi_buf[1], (const double*)c->psi_buf[2], c->psi_buf[3]);
+    if (c->Ny > 1)
+        hipLaunchKernelGGL(swm::psi_row<T>, dim3(segs, (unsigned)(c->Ny - 1)), dim3(256), 0, c->stream, sp, 1,
+                           (const double*)c->psi_buf[1], (const double*)c->psi_buf[2], c->psi_buf[3]);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_out) {
     HipLock lock(g_hip_mu);
     if (!c || !name || !host_out) return 1;
     if (int rc = member_refresh(c)) return rc;
     std::string s(name);
     for (auto& ch : s) ch = (char)toupper((unsigned char)ch);
+    if (s == "PSI") {   // computeStreamfunction of u, v at N0 (diagVar DVARNAME_PSI)
+        if (tracer_id >= 0) return fail(c, "PSI is not a tracer field", 1);
+        int rc = c->fp32 ? psi_t<float>(c) : psi_t<double>(c);
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpy2DAsync(host_out, c->Nx * 8, c->psi_buf[3], c->P * 8, c->Nx * 8, c->Ny,
+                                   hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return 0;
+    }
     if (s == "CHI") {   // the last device-resident SOR solution
         if (c->chi_last < 0) return fail(c, "no velocity potential has been computed", 1);
         const int w = c->chi_last;

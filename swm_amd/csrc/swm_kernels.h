@@ -1503,6 +1503,88 @@ __device__ __forceinline__ void nsum_add(double& s, double& c, double x) {
     s = t;
 }
 
+// computeStreamfunction (src/calc_lib.f90:727-780) of the master u, v at N0,
+// built without an elliptic solver (u_nd = u, 563-564).  Path integrals in
+// FP64, each SUM from its first element to its last as the reference's loop
+// runs it.  psi_prep writes the summands: pv = [H_v*]v, with the ocean_v
+// factor in row 1 (H_v*ocean_v*v); pu = [H_u*]u, with ocean_u in column Nx.
+template <typename T>
+__global__ void __launch_bounds__(256)
+psi_prep(const Statics<T>* sp, const T* u, const T* v, double* pu, double* pv) {
+    const Statics<T>& a = *sp;
+    const int Nx = a.Nx, Ny = a.Ny, P = a.P;
+    const long n = (long)Nx * Ny;
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= n) return;
+    const int j = (int)(idx / Nx), i = (int)(idx - (long)j * Nx);
+    const size_t off = (size_t)j * P + i;
+    const int m = a.mask[off];
+    const double ou = (m & OC_U) ? 1.0 : 0.0, ov = (m & OC_V) ? 1.0 : 0.0;
+    const double uu = (double)u[off], vv = (double)v[off];
+    const double hu = (double)fld(a.Hu, off, j, (m & OC_U) != 0), hv = (double)fld(a.Hv, off, j, (m & OC_V) != 0);
+    if (a.barotropic) {
+        pv[off] = j == 0 ? hv * ov * vv : hv * vv;
+        pu[off] = i == Nx - 1 ? hu * ou * uu : hu * uu;
+    } else {
+        pv[off] = j == 0 ? ov * vv : vv;
+        pu[off] = i == Nx - 1 ? ou * uu : uu;
+    }
+}
+
+// Column sums sum_{q < j} pu(i, q), one thread per column in order of j; in
+// column Nx they give psi(Nx, j) = -sum * A * dTheta (psi(Nx, 1) = 0).
+template <typename T>
+__global__ void __launch_bounds__(256)
+psi_col(const Statics<T>* sp, const double* pu, double* colsum, double* psi) {
+    const Statics<T>& a = *sp;
+    const int Nx = a.Nx, Ny = a.Ny, P = a.P;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= Nx) return;
+    const double A = (double)a.A, dT = (double)a.dT;
+    double s = 0.0;
+    for (int j = 0; j < Ny; ++j) {
+        const size_t off = (size_t)j * P + i;
+        colsum[off] = s;
+        if (i == Nx - 1) psi[off] = j == 0 ? 0.0 : -s * A * dT;
+        s = s + pu[off];
+    }
+}
+
+// Row sums sum_{q = i}^{Nx-1} pv(q, j) for 256 cells of one row per block,
+// each cell adding q = i, i+1, ... in order (rows staged through LDS).  Row 1
+// (j0 = 0) gives psi(i, 1); the interior combines both paths:
+// psi = (-S_row*A*cos_v*dLambda + psi(Nx, j) - S_col*A*dTheta + psi(i, 1)) / 2.
+template <typename T>
+__global__ void __launch_bounds__(256)
+psi_row(const Statics<T>* sp, int j0, const double* pv, const double* colsum, double* psi) {
+    constexpr int CH = 2048;
+    __shared__ double buf[CH];
+    const Statics<T>& a = *sp;
+    const int Nx = a.Nx, P = a.P;
+    const int j = j0 + blockIdx.y;
+    const int i0 = blockIdx.x * 256;
+    const int i = i0 + threadIdx.x;
+    const double* row = pv + (size_t)j * P;
+    double s = 0.0;
+    for (int c0 = i0; c0 < Nx - 1; c0 += CH) {
+        const int len = min(CH, Nx - 1 - c0);
+        __syncthreads();
+        for (int q = threadIdx.x; q < len; q += 256) buf[q] = row[c0 + q];
+        __syncthreads();
+        const int qs = max(i - c0, 0);
+        for (int q = qs; q < len; ++q) s = s + buf[q];
+    }
+    if (i >= Nx - 1) return;
+    const double A = (double)a.A, dL = (double)a.dL, dT = (double)a.dT;
+    const double cv = (double)a.cv[j];
+    const size_t off = (size_t)j * P + i;
+    if (j == 0) {
+        psi[off] = -s * A * cv * dL;
+    } else {
+        psi[off] = (-s * A * cv * dL + psi[(size_t)j * P + Nx - 1] - colsum[off] * A * dT + psi[i]) / 2.0;
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 sums_kernel(const Statics<T>* sp, int fn, const T* eta, const T* u, const T* v, const T* ch, const double* ce,

@@ -42,6 +42,25 @@ def linear_osc_setup(npz):
     return host.finalize(s)
 
 
+def diaglist_grid_setup(npz, variant=None):
+    """The test_diaglist domain (file bathymetry, H_overwrite = 3, BAROTROPIC)
+    with the linear D alias applied to the grid thicknesses."""
+    from swm_amd import host
+    variant = host.LINEARISED_STATE_OF_REST if variant is None else variant
+    p = json.loads(str(npz["params"]))
+    Hin = npz["H"].astype(np.float64)
+    miss = (npz["H"] == npz["H_missing"]).astype(np.int8)
+    grids, dL, dT = host.init_domain(int(p["domain_nl.nx"]), int(p["domain_nl.ny"]), p["domain_nl.lon_s"],
+                                     p["domain_nl.lon_e"], p["domain_nl.lat_s"], p["domain_nl.lat_e"], Hin, miss,
+                                     p["domain_nl.h_overwrite"], 0.0, 3, p["domain_nl.theta0"], p["domain_nl.omega"],
+                                     p["domain_nl.a"])
+    s = host.Setup(Nx=int(p["domain_nl.nx"]), Ny=int(p["domain_nl.ny"]), variant=variant, grids=grids,
+                   dLambda=dL, dTheta=dT, dt=float(p["model_nl.dt"]), A=float(p["domain_nl.a"]), lbc=0.0,
+                   barotropic=True)
+    host.apply_linear_D_alias(s)
+    return host.finalize(s)
+
+
 def same(a, b):
     """Bitwise-equal values (NaN positions must agree; +0 and -0 compare equal)."""
     na, nb = np.isnan(a), np.isnan(b)

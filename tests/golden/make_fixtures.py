@@ -80,6 +80,16 @@ def main():
         out[fname + "_missing"] = a["missing_value"]
     np.savez_compressed(os.path.join(HERE, "diaglist_static.npz"), H=H3, H_missing=np.float32(Ha3["missing_value"]),
                         params=params("test_diaglist"), **out)
+    # test_diaglist: recorded u, v and psi snapshots (psi is computeStreamfunction
+    # of the recorded u, v; three of the eleven snapshots)
+    snap = {}
+    for fname, var in (("u_out", "u"), ("v_out", "v"), ("psi_out", "psi")):
+        d, a, t = read_var(os.path.join(exp, f"output/reference_000000000001_{fname}.nc"), var)
+        snap[var] = d[[1, 5, 10]]
+        snap[var + "_missing"] = a["missing_value"]
+        snap["time"] = t[[1, 5, 10]]
+    np.savez_compressed(os.path.join(HERE, "diaglist_psi.npz"), H=H3, H_missing=np.float32(Ha3["missing_value"]),
+                        params=params("test_diaglist"), **snap)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -5,7 +5,7 @@ import json
 import numpy as np
 
 import oracle
-from conftest import linear_osc_setup, load_golden
+from conftest import diaglist_grid_setup, linear_osc_setup, load_golden
 from swm_amd import host
 
 
@@ -79,3 +79,52 @@ def test_diaglist_initial_H_and_impl():
         assert np.abs(arr[m] - r[m]).max() / np.abs(r[m]).max() < 1e-14
     r = npz["impl_n"]
     assert np.array_equal(r != npz["impl_n_missing"], s.grids["eta"].ocean == 1)
+
+
+def test_diaglist_psi_from_recorded_flow():
+    """computeStreamfunction (calc_lib.f90:727-780) of the recorded u, v gives
+    the recorded psi bit for bit (three snapshots of test_diaglist; land is
+    missing in the records and zero in the model)."""
+    npz = load_golden("diaglist_psi.npz")
+    s = diaglist_grid_setup(npz)
+    o = oracle.Oracle(s)
+    for k in range(npz["psi"].shape[0]):
+        u = np.where(npz["u"][k] == npz["u_missing"], 0.0, npz["u"][k])
+        v = np.where(npz["v"][k] == npz["v_missing"], 0.0, npz["v"][k])
+        ref = npz["psi"][k]
+        m = ref != npz["psi_missing"]
+        got = o.psi(u, v)
+        assert np.array_equal(got[m], ref[m]), k
+        assert np.abs(ref[m]).max() > 1e5
+    o.close()
+
+
+def test_psi_path_integrals_are_exact_for_a_discrete_streamfunction():
+    """A flow built from a streamfunction on the H grid as the reference
+    discretises it (u = -dpsi/dtheta / (A H_u), v = dpsi/dlambda / (A cos H_v))
+    integrates back to that psi to rounding, psi being zero on the boundary
+    paths and constant across land faces; a sign or index slip in either path
+    integral fails."""
+    npz = load_golden("diaglist_psi.npz")
+    s = diaglist_grid_setup(npz)
+    Ny, Nx = s.shape
+    rng = np.random.default_rng(4)
+    Hu, Hv = s.grids["u"].H, s.grids["v"].H
+    cv = s.grids["v"].cos_lat
+    psi0 = rng.standard_normal(s.shape)
+    # psi is constant across land faces (no flow there): zero it at every H
+    # point touching a face with zero thickness (u(i,j) and u(i,j-1) lie
+    # between H(i,j-1), H(i,j), H(i,j+1); v(i,j), v(i-1,j) likewise in i)
+    land_u, land_v = Hu <= 0, Hv <= 0
+    touch = land_u | np.roll(land_u, 1, axis=0) | land_v | np.roll(land_v, 1, axis=1)
+    touch[0, :] = touch[-1, :] = touch[:, 0] = touch[:, -1] = True
+    psi0[touch] = 0.0
+    # u(i,j) from psi(i,j+1) - psi(i,j), v(i,j) from psi(i+1,j) - psi(i,j)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        u = np.where(Hu > 0, -(np.roll(psi0, -1, axis=0) - psi0) / (s.A * s.dTheta * Hu), 0.0)
+        v = np.where(Hv > 0, (np.roll(psi0, -1, axis=1) - psi0) / (s.A * s.dLambda * cv[:, None] * Hv), 0.0)
+    o = oracle.Oracle(s)
+    got = o.psi(u, v)
+    o.close()
+    assert (psi0 != 0).sum() > 0.5 * psi0.size
+    assert np.abs(got - psi0).max() <= 1e-12 * np.abs(psi0).max()
