@@ -28,12 +28,12 @@
 extern "C" {
 #endif
 
-#define SWM_GPU_ABI_VERSION 1
+#define SWM_GPU_ABI_VERSION 2
 
 /* model.h physics variant (include/model.h:9-12 of each experiment) */
 enum swm_variant {
     SWM_FULLY_NONLINEAR = 1,
-    SWM_LINEARISED_MEAN_STATE = 2,          /* not supported on the GPU path: create fails */
+    SWM_LINEARISED_MEAN_STATE = 2,          /* needs u_bs, v_bs, zeta_bs and Ny <= Nx */
     SWM_LINEARISED_STATE_OF_REST = 3
 };
 /* model.h switches */
@@ -84,6 +84,10 @@ typedef struct swm_desc {
      * NULL = none.  Applied after windstress. */
     const double *osc_re[3], *osc_im[3];
     double osc_omega[3];
+    /* LINEARISED_MEAN_STATE basic state as SWM_timestep_init leaves it
+     * (src/swm_timestep_module.f90:76-87): u_bs, v_bs (evaluateStreamfunction of
+     * psi_bs) and zeta_bs (laplacian of psi_bs on the H grid); NULL otherwise */
+    const double *u_bs, *v_bs, *zeta_bs;
 } swm_desc;
 
 /* TRC_tracer, src/tracer_vars.f90:36-50 */

@@ -33,8 +33,10 @@ def _smooth_noise(rng, Ny, Nx, sigma):
 
 def basin(Nx, Ny, dx, dy, *, variant, H0, dt, G=9.81, Ah=0.0, lbc=2.0, f0=0.0, beta=0.0,
           drag=0.0, lateral_mixing=False, land_fraction=0.0, seed=0, A=A_FLAT,
-          quadratic_friction=False, gamma_sq=0.0):
-    """A closed flat-bottom basin of depth H0 with optional seeded islands."""
+          quadratic_friction=False, gamma_sq=0.0, psi_bs=None):
+    """A closed flat-bottom basin of depth H0 with optional seeded islands.
+    LINEARISED_MEAN_STATE takes its basic state from the streamfunction psi_bs
+    (zero if None)."""
     rng = np.random.default_rng(seed)
     lon_e = _extent_deg(Nx, dx, A)
     lat_e = _extent_deg(Ny, dy, A)
@@ -57,6 +59,8 @@ def basin(Nx, Ny, dx, dy, *, variant, H0, dt, G=9.81, Ah=0.0, lbc=2.0, f0=0.0, b
     s = H.Setup(Nx=Nx, Ny=Ny, variant=variant, grids=grids, dLambda=dL, dTheta=dT,
                 lateral_mixing=lateral_mixing, quadratic_friction=quadratic_friction, barotropic=True,
                 G=G, Ah=Ah, dt=dt, A=A, RHO0=1024.0, lbc=lbc)
+    if variant == H.LINEARISED_MEAN_STATE:
+        H.set_mean_state(s, np.zeros((Ny, Nx)) if psi_bs is None else psi_bs)
     H.apply_linear_D_alias(s)
     terms = []
     if drag:
@@ -183,4 +187,23 @@ def serve_member(seed: int, Nx: int, Ny: int, H0: float, Ah: float, tau0: float,
     if tracer:
         H.add_tracer(s, fourier_modes(s, s._rng), kappa_h=50.0)
     s.Nt = nsteps
+    return H.finalize(s)
+
+
+def mean_state_basin(seed=0, Nx=96, Ny=64, dx=10e3, dt=20.0, H0=100.0, U=0.1, lateral_mixing=True, Ah=200.0,
+                     land_fraction=0.05):
+    """LINEARISED_MEAN_STATE basin: a perturbation on a steady basic flow of
+    speed ~U from a smooth seeded streamfunction (zero on land), f-plane plus
+    beta, linear drag, double-gyre windstress, a Gaussian eta bump."""
+    rng = np.random.default_rng(seed)
+    stub = basin(Nx, Ny, dx, dx, variant=H.LINEARISED_STATE_OF_REST, H0=H0, dt=dt, seed=seed,
+                 land_fraction=land_fraction)
+    psi = fourier_modes(stub, rng, n_modes=6, kmax=3)
+    psi = psi * (U * H0 * dx * 4.0 / np.abs(psi).max()) * stub.grids["H"].ocean
+    s = basin(Nx, Ny, dx, dx, variant=H.LINEARISED_MEAN_STATE, H0=H0, dt=dt, f0=1e-4, beta=2e-11, drag=1e-6,
+              lateral_mixing=lateral_mixing, Ah=Ah, lbc=0.0, land_fraction=land_fraction, seed=seed, psi_bs=psi)
+    H.add_windstress(s, H.COMP_X, double_gyre_tau(s, 0.05))
+    x = (np.arange(Nx) - Nx / 2) * dx
+    y = (np.arange(Ny) - Ny / 2) * dx
+    s.eta0 = 0.1 * np.exp(-(x[None, :] ** 2 + y[:, None] ** 2) / (2 * (8 * dx) ** 2))
     return H.finalize(s)

@@ -51,7 +51,7 @@ constexpr int kMaxTracers = 2;
 
 enum FieldId {
     F_HE, F_HU, F_HV, F_IMU, F_IMV, F_IME, F_GSU, F_GSV, F_FXC, F_FYC, F_FEC, F_TAUX, F_TAUY,
-    F_ORX, F_OIX, F_ORY, F_OIY, F_ORE, F_OIE, F_COUNT
+    F_ORX, F_OIX, F_ORY, F_OIY, F_ORE, F_OIE, F_UBS, F_VBS, F_ZBS, F_COUNT
 };
 enum TabId {
     T_CU, T_CV, T_CE, T_CH, T_FH, T_ZA, T_EDU, T_ACE, T_LMU, T_LMV, T_PLLC, T_PLTC, T_TA, T_TB, T_TC,
@@ -188,6 +188,8 @@ struct swm_ctx {
     int cur = 0;                 // ping-pong index of level N0
     int ring = 0;                // slot holding G(1) (oldest); G(2) = ring+1, newest = ring+2
     int64_t itt = 0;
+    int64_t itt_upload = 0;      // itt of the last state hand-over: the other level holds the
+                                 // state before the last step only when itt > itt_upload
     uint8_t* mask = nullptr;
     std::vector<int8_t> hmask[4];
     FieldDev fields[F_COUNT];
@@ -475,6 +477,7 @@ int write_statics_t(swm_ctx* c) {
     a.oRx = F<T>(c->fields[F_ORX]); a.oIx = F<T>(c->fields[F_OIX]);
     a.oRy = F<T>(c->fields[F_ORY]); a.oIy = F<T>(c->fields[F_OIY]);
     a.oRe = F<T>(c->fields[F_ORE]); a.oIe = F<T>(c->fields[F_OIE]);
+    a.ubs = F<T>(c->fields[F_UBS]); a.vbs = F<T>(c->fields[F_VBS]); a.zbs = F<T>(c->fields[F_ZBS]);
     a.cu = (const T*)c->tab[T_CU]; a.cv = (const T*)c->tab[T_CV]; a.ce = (const T*)c->tab[T_CE];
     a.cH = (const T*)c->tab[T_CH]; a.fH = (const T*)c->tab[T_FH]; a.za = (const T*)c->tab[T_ZA];
     a.edu = (const T*)c->tab[T_EDU]; a.ace = (const T*)c->tab[T_ACE];
@@ -594,7 +597,9 @@ void launch_tile(swm_ctx* c, const Dyn<T>& d) {
     dim3 grid(d.nblk), block(256);
 #define SWM_LAUNCH(V, N) \
     hipLaunchKernelGGL((swm::step_kernel<T, V, N, TX, TY, DIAG>), grid, block, 0, c->stream, d)
-    if (var == SWM_FULLY_NONLINEAR) {
+    if (var == SWM_LINEARISED_MEAN_STATE) {   // no tracers on the linear variants
+        SWM_LAUNCH(swm::VAR_MS, 0);
+    } else if (var == SWM_FULLY_NONLINEAR) {
         if (nt == 0) SWM_LAUNCH(swm::VAR_FN, 0);
         else if (nt == 1) SWM_LAUNCH(swm::VAR_FN, 1);
         else SWM_LAUNCH(swm::VAR_FN, 2);
@@ -768,8 +773,10 @@ int step_many(swm_ctx* c, int64_t nsteps) {
     return 0;
 }
 
+// prev: evaluate on the state before the last step (the other ping-pong level),
+// which is what the lateral-mixing register holds after a step.
 template <typename T>
-int materialise(swm_ctx* c) {
+int materialise(swm_ctx* c, bool prev = false) {
     bool fresh = false;
     for (int k = 0; k < 14; ++k)
         if (!c->diag[k]) {
@@ -787,6 +794,7 @@ int materialise(swm_ctx* c) {
         if (rc) return rc;
     }
     Dyn<T> d = make_dyn<T>(c, 2);   // the materialisation kernel uses the 64 x 8 tile (kTiles[2])
+    if (prev) d.cur = 1 - c->cur;
     // forcing as left by the last SWM_forcing_update
     set_osc(c, d, (double)c->itt);
     set_streams(c, d, (double)c->itt);
@@ -825,9 +833,22 @@ int swm_gpu_create(const swm_desc* d, swm_ctx** out) {
     if (!d) { g_create_err = "null descriptor"; return 1; }
     if (d->abi_version != SWM_GPU_ABI_VERSION) { g_create_err = "ABI version mismatch"; return 1; }
     if (d->Nx < 3 || d->Ny < 3) { g_create_err = "Nx and Ny must be >= 3"; return 1; }
-    if (d->variant != SWM_FULLY_NONLINEAR && d->variant != SWM_LINEARISED_STATE_OF_REST) {
-        g_create_err = "variant not supported on the GPU path (FULLY_NONLINEAR, LINEARISED_STATE_OF_REST)";
+    if (d->variant != SWM_FULLY_NONLINEAR && d->variant != SWM_LINEARISED_STATE_OF_REST &&
+        d->variant != SWM_LINEARISED_MEAN_STATE) {
+        g_create_err = "unknown variant";
         return 4;
+    }
+    if (d->variant == SWM_LINEARISED_MEAN_STATE) {
+        if (!d->u_bs || !d->v_bs || !d->zeta_bs) {
+            g_create_err = "LINEARISED_MEAN_STATE needs the basic state u_bs, v_bs and zeta_bs";
+            return 1;
+        }
+        // the G_eta basic-state term reads column ip1(j) of row j
+        // (src/swm_timestep_module.f90:360); ip1 has Nx entries
+        if (d->Ny > d->Nx) {
+            g_create_err = "LINEARISED_MEAN_STATE needs Ny <= Nx (the reference indexes ip1(j))";
+            return 4;
+        }
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_create_err = "no HIP device"; return 5; }
@@ -854,6 +875,7 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
     // longer 512-byte rows of 64x4 tiles win.
     c->tile = (c->fp32 || (long long)d->Nx * d->Ny < 3072LL * 3072LL) ? 1 : 0;
     if (const char* e = getenv("SWM_GPU_TILE")) c->tile = std::max(0, std::min(kNumTiles - 1, atoi(e)));
+    if (d->variant == SWM_LINEARISED_MEAN_STATE && kTiles[c->tile].march) c->tile = 1;   // LDS tile kernels only
     if (const char* e = getenv("SWM_GPU_GRAPH")) c->use_graph = atoi(e) != 0;
     const int align = c->fp32 ? 32 : 16;   // 128-byte aligned rows
     c->P = (c->Nx + align - 1) / align * align;
@@ -910,7 +932,8 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
         {F_TAUY, d->tau_y, 0.0, mv},              {F_ORX, d->osc_re[0], 0.0, mu},
         {F_OIX, d->osc_im[0], 0.0, mu},           {F_ORY, d->osc_re[1], 0.0, mv},
         {F_OIY, d->osc_im[1], 0.0, mv},           {F_ORE, d->osc_re[2], 0.0, me},
-        {F_OIE, d->osc_im[2], 0.0, me}};
+        {F_OIE, d->osc_im[2], 0.0, me},           {F_UBS, d->u_bs, 0.0, mu},
+        {F_VBS, d->v_bs, 0.0, mv},                {F_ZBS, d->zeta_bs, 0.0, c->hmask[0].data()}};
     for (auto& u : ups)
         if ((rc = upload_field(c, &c->fields[u.id], u.h, u.dflt, u.m))) return rc;
     if ((rc = build_tables(c))) return rc;
@@ -921,6 +944,7 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
     for (int g = 0; g < 4; ++g) c->d.grid[g] = swm_grid_desc{};
     c->d.impl_u = c->d.impl_v = c->d.impl_eta = c->d.gamma_sq_u = c->d.gamma_sq_v = nullptr;
     c->d.F_x_const = c->d.F_y_const = c->d.F_eta_const = c->d.tau_x = c->d.tau_y = nullptr;
+    c->d.u_bs = c->d.v_bs = c->d.zeta_bs = nullptr;
     for (int k = 0; k < 3; ++k) {
         c->d.osc_re[k] = d->osc_re[k] ? (const double*)1 : nullptr;   // presence flag only
         c->d.osc_im[k] = nullptr;
@@ -959,6 +983,7 @@ int swm_gpu_upload_state(swm_ctx* c, const double* u, const double* v, const dou
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->itt = itt;
+    c->itt_upload = itt;
     return 0;
 }
 
@@ -1127,7 +1152,19 @@ int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_o
         for (int q = 0; q < 14; ++q)
             if (s == names[q]) k = q;
         if (k < 0) return fail(c, "unknown field " + s, 1);
-        int rc = c->fp32 ? materialise<float>(c) : materialise<double>(c);
+        if (k >= 12) {
+            // swm_latmix_u/v are computed at the start of SWM_timestep_step from the
+            // state before the step (src/swm_timestep_module.f90:171-173) and hold
+            // initVar's 0 before the first step.
+            if (c->itt == c->itt_upload && c->itt > 0)
+                return fail(c, "the lateral-mixing trend of the last step is not on the device after a "
+                               "mid-run hand-over; step once first", 1);
+            if (c->itt == 0 || !(c->d.flags & SWM_LATERAL_MIXING)) {
+                for (size_t q = 0; q < (size_t)c->Nx * c->Ny; ++q) host_out[q] = 0.0;
+                return 0;
+            }
+        }
+        int rc = c->fp32 ? materialise<float>(c, k >= 12) : materialise<double>(c, k >= 12);
         if (rc) return rc;
         src = c->diag[k];
     }
@@ -1354,6 +1391,8 @@ int swm_gpu_sor_fixed(swm_ctx* c, const double* B, double* chi, double omega, do
 int swm_gpu_set_kernel_shape(swm_ctx* c, int32_t shape) {
     if (!c) return 1;
     if (shape < 0 || shape >= kNumTiles) return fail(c, "no such kernel shape", 1);
+    if (c->d.variant == SWM_LINEARISED_MEAN_STATE && kTiles[shape].march)
+        return fail(c, "LINEARISED_MEAN_STATE runs on the LDS tile kernels (shapes 0-4)", 4);
     HipLock lock(g_hip_mu);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     drop_graphs(c);
@@ -1427,6 +1466,10 @@ int swm_gpu_tune(swm_ctx* c, int32_t steps, int32_t* best_shape, double* times_m
     if (hipMemcpyAsync(snap, c->state, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) rc = 3;
     c->use_graph = false;   // time the kernels, not graph capture
     for (int k = 0; k < kNumTiles && !rc; ++k) {
+        if (c->d.variant == SWM_LINEARISED_MEAN_STATE && kTiles[k].march) {
+            if (times_ms) times_ms[k] = -1.0;   // not available for this variant
+            continue;
+        }
         c->tile = k;
         float ms = 0.f;
         // untimed run first (first-launch costs), then the timed run from the same state
@@ -1787,6 +1830,8 @@ int swm_gpu_batch_attach(swm_batch* b, swm_ctx* c, int64_t nsteps) {
     if (c->batch) return bfail(b, "context is already attached to a batch", 1);
     if (nsteps < 0) return bfail(b, "nsteps must be >= 0", 1);
     if (c->has_osc) return bfail(b, "oscillating forcing needs per-step host phases; step such a context alone", 4);
+    if (c->d.variant == SWM_LINEARISED_MEAN_STATE)
+        return bfail(b, "LINEARISED_MEAN_STATE contexts are stepped alone", 4);
     if (!c->streams.empty())
         return bfail(b, "time-interpolated forcing streams need per-step host arguments; step such a context alone", 4);
     if (!c->avgs.empty()) return bfail(b, "averaging buffers are kept by contexts stepped alone", 4);

@@ -21,7 +21,7 @@
 namespace swm {
 
 enum : uint8_t { OC_H = 1, OC_U = 2, OC_V = 4, OC_E = 8 };
-enum { VAR_FN = 1, VAR_SOR = 3 };
+enum { VAR_FN = 1, VAR_MS = 2, VAR_SOR = 3 };
 
 // A static per-cell input: either a full (Ny x P) array or, when every row holds
 // one value on the grid's ocean points and one on its land points, two values
@@ -62,6 +62,8 @@ struct Statics {
     // tracers
     T kappa[2];
     Fld<T> C0[2], gC[2], timpl[2];
+    // LINEARISED_MEAN_STATE basic state u_bs, v_bs (u, v grids) and zeta_bs (H grid)
+    Fld<T> ubs, vbs, zbs;
     // time-interpolated forcing streams: kind 0 = windstress, 1 = custom or heating
     // (added as is); component 0 = x (ocean u), 1 = y (ocean v), 2 = eta
     int nstream, sdiv;   // sdiv: windstress divides by RHO0 * D_face (BAROTROPIC or FULLY_NONLINEAR)
@@ -196,9 +198,10 @@ __device__ __forceinline__ T trc_adv_r(bool neg, int nummask, int den, const T* 
 // are recomputed from D, u, v where they are needed, which keeps the FP64
 // tile under 32 KiB (five blocks per CU).  NTR is the largest tracer count the
 // kernel handles.
-template <typename T, int NTR, int E, int EH>
+template <typename T, int NTR, int E, int EH, int ZE = 1>
 struct TileLds {
     T u[E], v[E], D[E], Pot[E], Ed[E], pll[E], plt[E];   // Pot holds eta until phase 2 replaces it
+    T zt[ZE], et[ZE];    // LINEARISED_MEAN_STATE (ZE = E): relative vorticity and eta for phase 3
     T CH[NTR][E], C[NTR][E];
     // per-row tables of the EH tile rows (cu also of the row above them)
     T za[EH][2], cu[EH + 1], cH[EH], fH[EH], pltc[EH][12], pllc[EH][12];
@@ -219,6 +222,13 @@ __device__ __forceinline__ void gst(T* p, size_t i, T v) {
 }
 template <typename T>
 using CStatics = const __attribute__((address_space(4))) Statics<T>;
+
+// A static field at element offset `at` of row `row` (full array, or the
+// row's ocean / land value), read through the global address space.
+template <typename F>
+__device__ __forceinline__ auto fldg(const F& f, size_t at, int row, bool ocean) {
+    return f.full ? gld(f.full, at) : gld(f.rows, 2 * (size_t)row + (ocean ? 0 : 1));
+}
 
 // Time-interpolated forcing streams, after the descriptor's windstress and
 // oscillating streams: SWM_forcing_processWindstress / processCustomForcing /
@@ -441,6 +451,7 @@ __device__ __forceinline__ void tile_stage(const Dyn<T>& d, const int t, const R
             s_u[e] = ru[q];
             s_v[e] = rv[q];
             s_eta[e] = eta;
+            if constexpr (VAR == VAR_MS) lds.et[e] = eta;
             const T He = OE(m) ? rHo[q] : rHl[q];
             T D;
             if (FN) D = OE(m) ? (eta + He) : T(1);   // computeD: D = eta + H on ocean eta, 1 elsewhere
@@ -538,7 +549,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             // computeRelVort -> vorticityFromVelocities2D (src/calc_lib.f90:663-678,
             // pder_zonal3D 892-926, pder_meridional3D 965-998)
             T zeta = T(0);
-            if (FN) {
+            if (FN || VAR == VAR_MS) {
                 const int gjm = gj == 0 ? Ny - 1 : gj - 1;
                 T za_ = T(0), zb = T(0);
                 if (OV(s_m[kw]) + OV(m) != 0)
@@ -551,7 +562,14 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             }
             // computePotVort (src/swm_timestep_module.f90:230-257)
             const T pD = Dh > a.eps ? Dh : a.eps;
-            const T Pot = FN ? dv(lds.fH[ly] + zeta, pD) : dv(lds.fH[ly], pD);
+            T Pot;
+            if constexpr (VAR == VAR_MS) {   // (f + zeta_bs) / pD
+                const int gi = wrap(i0 + lx - 1, Nx);
+                Pot = dv(lds.fH[ly] + fldg(a.zbs, (size_t)gj * P + gi, gj, OH(m)), pD);
+                lds.zt[e] = zeta;
+            } else {
+                Pot = FN ? dv(lds.fH[ly] + zeta, pD) : dv(lds.fH[ly], pD);
+            }
             s_Pot[e] = Pot;
             // SWM_LateralMixing_compute_p, P_lambda-theta on ocean H points
             T plt = T(0);
@@ -578,6 +596,16 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                 if (FN) {
                     const T ue = s_u[ke], uc = s_u[e], vn = s_v[kn], vc = s_v[e];
                     Ed = ((T(0.5) * (ue * ue) + T(0.5) * (uc * uc)) + (T(0.5) * (vn * vn) + T(0.5) * (vc * vc))) / T(2) +
+                         a.G * eta_e;
+                } else if (VAR == VAR_MS) {   // g*eta + uU + vV
+                    const int gi = wrap(i0 + lx - 1, Nx), gip = gi + 1 == Nx ? 0 : gi + 1;
+                    const int gjp = gj + 1 == Ny ? 0 : gj + 1;
+                    const T ube = fldg(a.ubs, (size_t)gj * P + gip, gj, OU(s_m[ke]) != 0);
+                    const T ubc = fldg(a.ubs, (size_t)gj * P + gi, gj, OU(m) != 0);
+                    const T vbn = fldg(a.vbs, (size_t)gjp * P + gi, gjp, OV(s_m[kn]) != 0);
+                    const T vbc = fldg(a.vbs, (size_t)gj * P + gi, gj, OV(m) != 0);
+                    const T ue = s_u[ke], uc = s_u[e], vn = s_v[kn], vc = s_v[e];
+                    Ed = ((T(0.5) * (ue * ube) + T(0.5) * (uc * ubc)) + (T(0.5) * (vn * vbn) + T(0.5) * (vc * vbc))) +
                          a.G * eta_e;
                 } else {
                     Ed = a.G * eta_e;
@@ -666,15 +694,44 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
 
         // SWM_timestep_nonlinear (src/swm_timestep_module.f90:349-443)
         T Ge = T(0), Gu = T(0), Gv = T(0), Lu = T(0), Lv = T(0);
+        // neighbours' offsets for the basic-state terms (LINEARISED_MEAN_STATE)
+        const int ipx = i + 1 == Nx ? 0 : i + 1, imx = i == 0 ? Nx - 1 : i - 1, jmx = j == 0 ? Ny - 1 : j - 1;
         if (oe) {
             const T t1 = dv(-(MUe - MUk), a.dL);
             const T t2 = dv(MVn * rt(a.cv, jp) - MVk * rt(a.cv, j), a.dT);
-            Ge = dv(t1 - t2, rt(a.ace, j)) + Fe;
+            T tt = t1 - t2;
+            if constexpr (VAR == VAR_MS) {
+                // src/swm_timestep_module.f90:359-365.  The first interpolation is
+                // taken at i index ip1(j) (column j+1 of row j), as the reference
+                // indexes it; rows j < Nx only (create rejects Ny > Nx).
+                const int c = j + 1 == Nx ? 0 : j + 1, cm = c == 0 ? Nx - 1 : c - 1;
+                const size_t rc = (size_t)j * P;
+                const T E1 = interp_noland2(gld(se, rc + c), gld(se, rc + cm), OE(gld(a.mask, rc + c)),
+                                            OE(gld(a.mask, rc + cm)));
+                const T E0 = interp_noland2(lds.et[k], lds.et[kw], oe, OE(s_m[kw]));
+                const T ube = fldg(a.ubs, (size_t)j * P + ipx, j, OU(s_m[ke]) != 0);
+                const T ubk = fldg(a.ubs, off, j, ou != 0);
+                tt = tt - dv(E1 * ube - E0 * ubk, a.dL);
+                const T E2 = interp_noland2(lds.et[kn], lds.et[k], OE(s_m[kn]), oe);
+                const T E3 = interp_noland2(lds.et[k], lds.et[ks], oe, OE(s_m[ks]));
+                const T vbn = fldg(a.vbs, (size_t)jp * P + i, jp, OV(s_m[kn]) != 0);
+                const T vbk = fldg(a.vbs, off, j, ov != 0);
+                tt = tt - dv((rt(a.cv, jp) * E2) * vbn - (rt(a.cv, j) * E3) * vbk, a.dT);
+            }
+            Ge = dv(tt, rt(a.ace, j)) + Fe;
         }
         if (ou) {
             const T P2 = T(0.5) * s_Pot[kn] + T(0.5) * s_Pot[k];
             const T M4 = ((T(0.25) * MVn + T(0.25) * MVw) + T(0.25) * MVnw) + T(0.25) * MVk;
             T g = P2 * M4;
+            if constexpr (VAR == VAR_MS) {   // + interp2(zeta, H2u) * interp4(v_bs, v2u)
+                const T Z2 = T(0.5) * lds.zt[kn] + T(0.5) * lds.zt[k];
+                const T vbn = fldg(a.vbs, (size_t)jp * P + i, jp, OV(s_m[kn]) != 0);
+                const T vbw = fldg(a.vbs, (size_t)j * P + imx, j, OV(s_m[kw]) != 0);
+                const T vbnw = fldg(a.vbs, (size_t)jp * P + imx, jp, OV(s_m[knw]) != 0);
+                const T vbk = fldg(a.vbs, off, j, ov != 0);
+                g = g + Z2 * (((T(0.25) * vbn + T(0.25) * vbw) + T(0.25) * vbnw) + T(0.25) * vbk);
+            }
             g = g - dv(s_Ed[k] - s_Ed[kw], rt(a.edu, j));
             if (a.quad) {
                 const T vi = ((T(0.25) * s_v[kn] + T(0.25) * s_v[kw]) + T(0.25) * s_v[knw]) + T(0.25) * s_v[k];
@@ -697,6 +754,14 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             const T P2 = T(0.5) * s_Pot[ke] + T(0.5) * s_Pot[k];
             const T M4 = ((T(0.25) * MUe + T(0.25) * MUs) + T(0.25) * MUk) + T(0.25) * MUse;
             T g = -(P2 * M4);
+            if constexpr (VAR == VAR_MS) {   // - interp2(zeta, H2v) * interp4(u_bs, u2v)
+                const T Z2 = T(0.5) * lds.zt[ke] + T(0.5) * lds.zt[k];
+                const T ube = fldg(a.ubs, (size_t)j * P + ipx, j, OU(s_m[ke]) != 0);
+                const T ubs_ = fldg(a.ubs, (size_t)jmx * P + i, jmx, OU(s_m[ks]) != 0);
+                const T ubk = fldg(a.ubs, off, j, ou != 0);
+                const T ubse = fldg(a.ubs, (size_t)jmx * P + ipx, jmx, OU(s_m[kse]) != 0);
+                g = g - Z2 * (((T(0.25) * ube + T(0.25) * ubs_) + T(0.25) * ubk) + T(0.25) * ubse);
+            }
             g = g - dv(s_Ed[k] - s_Ed[ks], a.AdT);
             if (a.quad) {
                 const T ui = ((T(0.25) * s_u[ke] + T(0.25) * s_u[ks]) + T(0.25) * s_u[k]) + T(0.25) * s_u[kse];
@@ -829,7 +894,8 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
 template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG>
 __global__ void __launch_bounds__(256)
 step_kernel(const Dyn<T> d) {
-    __shared__ TileLds<T, (NTRC > 0 ? NTRC : 1), (TX + 2) * (TY + 2), TY + 2> lds;
+    __shared__ TileLds<T, (NTRC > 0 ? NTRC : 1), (TX + 2) * (TY + 2), TY + 2,
+                       (VAR == VAR_MS ? (TX + 2) * (TY + 2) : 1)> lds;
     tile_body<T, VAR, NTRC, TX, TY, DIAG>(d, xcd_tile(blockIdx.x, d.nblk), lds);
 }
 

@@ -11,8 +11,9 @@ module swm_gpu_shim
   implicit none
   private
 
-  integer(c_int32_t), parameter, public :: SWM_GPU_ABI_VERSION = 1
-  integer(c_int32_t), parameter, public :: SWM_FULLY_NONLINEAR = 1, SWM_LINEARISED_STATE_OF_REST = 3
+  integer(c_int32_t), parameter, public :: SWM_GPU_ABI_VERSION = 2
+  integer(c_int32_t), parameter, public :: SWM_FULLY_NONLINEAR = 1, SWM_LINEARISED_MEAN_STATE = 2, &
+                                           SWM_LINEARISED_STATE_OF_REST = 3
   integer(c_int32_t), parameter, public :: SWM_LATERAL_MIXING = 1, SWM_QUADRATIC_BOTTOM_FRICTION = 2, &
                                            SWM_BAROTROPIC = 4, SWM_TAU_SCALE = 8
   integer(c_int32_t), parameter, public :: SWM_FP64 = 0, SWM_FP32 = 1
@@ -41,6 +42,7 @@ module swm_gpu_shim
     type(c_ptr) :: tau_x = c_null_ptr, tau_y = c_null_ptr
     type(c_ptr) :: osc_re(3) = c_null_ptr, osc_im(3) = c_null_ptr
     real(c_double) :: osc_omega(3) = 0
+    type(c_ptr) :: u_bs = c_null_ptr, v_bs = c_null_ptr, zeta_bs = c_null_ptr
   end type swm_desc
 
   !> swm_tracer_desc

@@ -21,7 +21,7 @@ from . import host as _host
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libswmgpu.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 FULLY_NONLINEAR, LINEARISED_MEAN_STATE, LINEARISED_STATE_OF_REST = 1, 2, 3
 LATERAL_MIXING, QUADRATIC_BOTTOM_FRICTION, BAROTROPIC, TAU_SCALE = 1, 2, 4, 8
@@ -58,6 +58,7 @@ class Desc(C.Structure):
         ("impl_u", _dp), ("impl_v", _dp), ("impl_eta", _dp), ("gamma_sq_u", _dp), ("gamma_sq_v", _dp),
         ("F_x_const", _dp), ("F_y_const", _dp), ("F_eta_const", _dp), ("tau_x", _dp), ("tau_y", _dp),
         ("osc_re", _dp * 3), ("osc_im", _dp * 3), ("osc_omega", C.c_double * 3),
+        ("u_bs", _dp), ("v_bs", _dp), ("zeta_bs", _dp),
     ]
 
 
@@ -231,6 +232,8 @@ class SwmGpu:
             d.osc_re[comp] = _arr(re, keep)
             d.osc_im[comp] = _arr(im, keep)
             d.osc_omega[comp] = om
+        if s.variant == _host.LINEARISED_MEAN_STATE:
+            d.u_bs, d.v_bs, d.zeta_bs = _arr(s.u_bs, keep), _arr(s.v_bs, keep), _arr(s.zeta_bs, keep)
         self._desc = d
         h = C.c_void_p()
         rc = self.L.swm_gpu_create(C.byref(d), C.byref(h))

@@ -275,6 +275,42 @@ class Setup:
 # swm_vars / computeD aliasing for the linear variants
 # ---------------------------------------------------------------------------
 
+def set_mean_state(s: Setup, psi_bs: np.ndarray) -> None:
+    """LINEARISED_MEAN_STATE basic state from its streamfunction on the H grid
+    (SWM_timestep_init, swm_timestep_module.f90:76-87): u_bs = evSF_zonal,
+    v_bs = evSF_meridional (calc_lib.f90:840-880), zeta_bs = laplacian2D
+    (calc_lib.f90:1170-1206, the `vorticity` of a scalar).  Evaluated with the
+    grid thicknesses as initDomain leaves them, i.e. before
+    apply_linear_D_alias, as the reference computes it before computeD."""
+    psi = np.asarray(psi_bs, dtype=np.float64)
+    A, dL, dT = s.A, s.dLambda, s.dTheta
+    gH, gu, gv = s.grids["H"], s.grids["u"], s.grids["v"]
+    oH = gH.ocean.astype(np.int64)
+    ip = lambda a: np.roll(a, -1, axis=1)     # a(ip1(i), j)
+    im = lambda a: np.roll(a, 1, axis=1)      # a(im1(i), j)
+    jp = lambda a: np.roll(a, -1, axis=0)     # a(i, jp1(j))
+    jm = lambda a: np.roll(a, 1, axis=0)      # a(i, jm1(j))
+    bc_u, bc_v = gu.bc(s.lbc), gv.bc(s.lbc)
+    # pder_meridional3D(psi, H_grid): out grid u, ind0 = jp1, indm1 = jp0
+    dth = np.where(jp(oH) + oH == 0, 0.0, bc_u * (jp(psi) - psi) / (A * dT))
+    u_bs = -dth
+    # pder_zonal3D(psi, H_grid): out grid v, ind0 = ip1, indm1 = ip0
+    cvr = gv.cos_lat[:, None]
+    dla = np.where(ip(oH) + oH == 0, 0.0, bc_v / (A * cvr * dL) * (ip(psi) - psi))
+    v_bs = dla
+    if s.barotropic:
+        with np.errstate(divide="ignore", invalid="ignore"):
+            u_bs = np.where(gu.ocean == 1, u_bs / gu.H, u_bs)
+            v_bs = np.where(gv.ocean == 1, v_bs / gv.H, v_bs)
+    # laplacian2D(psi, H_grid): lambda2 -> v grid (ip0, im1), theta2 -> u grid (jp0, jm1)
+    cH = gH.cos_lat[:, None]
+    cHp, cHm = np.roll(gH.cos_lat, -1)[:, None], np.roll(gH.cos_lat, 1)[:, None]
+    t1 = 1.0 / (A * cH * dL) ** 2 * (bc_v * (ip(psi) - psi) - im(bc_v) * (psi - im(psi)))
+    t2 = 1.0 / (A ** 2 * cH * dT ** 2) * (bc_u * (cHp * jp(psi) - cH * psi) - jm(bc_u) * (cH * psi - cHm * jm(psi)))
+    zeta = np.where(gH.ocean == 1, t1 + t2, 0.0)
+    s.u_bs, s.v_bs, s.zeta_bs = u_bs, v_bs, zeta
+
+
 def apply_linear_D_alias(s: Setup) -> None:
     """For the linearised variants D, Dh, Du, Dv are pointers to the grid H
     arrays (swm_vars.f90:93-105) and computeD at itt=0 overwrites H_grid%H,
@@ -456,6 +492,13 @@ def finalize(s: Setup) -> Setup:
         init_damping(s, [])
     if s.quadratic_friction and s.gamma_sq_u is None:
         s.gamma_sq_u, s.gamma_sq_v = z(), z()
+    if s.variant == LINEARISED_MEAN_STATE:
+        if s.Ny > s.Nx:
+            raise ValueError("LINEARISED_MEAN_STATE needs Ny <= Nx: the reference indexes ip1(j) "
+                             "(swm_timestep_module.f90:360) and ip1 has Nx entries")
+        for name in ("u_bs", "v_bs", "zeta_bs"):
+            if getattr(s, name) is None:
+                setattr(s, name, z())
     for a in ("u0", "v0", "eta0"):
         if getattr(s, a) is None:
             setattr(s, a, z())
@@ -557,6 +600,10 @@ def setup_from_selfcheck(exp_dir: str, defines: dict) -> Setup:
               dt=float(model_nl.get("dt", 10.0)), A=A, RHO0=float(domain_nl.get("rho0", 1024.0)),
               lbc=lbc, ab_chi=float(g1("tint_nl").get("ab_chi", 0.1)))
     s.Nt = int(float(model_nl.get("run_length", 100000.0)) / s.dt)
+    if variant == LINEARISED_MEAN_STATE:
+        ts_nl = g1("swm_timestep_nl")
+        psi_bs, _ = _read_records(path(ts_nl["filename"]), ts_nl["varname"])
+        set_mean_state(s, psi_bs[0])      # getChunkData(SWM_MC_bs_psi, 0): the first record
     apply_linear_D_alias(s)
     loader = lambda fn, vn: _read_nc(path(fn), vn)[0]
     init_damping(s, _nml.groups(nml, "swm_damping_nl"), loader)

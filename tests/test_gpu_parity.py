@@ -18,12 +18,14 @@ STATE = [("SWM_U", "u"), ("SWM_V", "v"), ("SWM_ETA", "eta"), ("G_U", "G_u"), ("G
          ("G_U_M1", "G_u_m1"), ("G_V_M1", "G_v_m1"), ("G_ETA_M1", "G_eta_m1")]
 DERIVED = [("SWM_D", "D"), ("SWM_DH", "Dh"), ("SWM_DU", "Du"), ("SWM_DV", "Dv"), ("SWM_RELVORT", "zeta"),
            ("SWM_POT", "Pot"), ("SWM_EDENS", "EDens"), ("SWM_MU", "MU"), ("SWM_MV", "MV"), ("F_X", "F_x"),
-           ("F_Y", "F_y"), ("F_ETA", "F_eta")]
+           ("F_Y", "F_y"), ("F_ETA", "F_eta"), ("SWM_LATMIX_U", "latmix_u"), ("SWM_LATMIX_V", "latmix_v")]
 TRACER = [("CH", "CH"), ("G_CH", "G_CH"), ("G_CH_M1", "G_CH_m1"), ("C", "C"), ("uhc", "uhc"), ("vhc", "vhc")]
 
 
 def compare(g, o, names, ntrc=0, tag=""):
     for gn, on in names:
+        if gn.startswith("SWM_LATMIX") and not g.s.lateral_mixing:
+            continue   # registered only with LATERAL_MIXING
         a, b = g.get(gn), o.get(on)
         assert same(a, b), f"{tag} {gn}: max |diff| {np.nanmax(np.abs(a - b))}"
     for t in range(ntrc):
