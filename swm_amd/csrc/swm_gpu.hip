@@ -41,11 +41,13 @@ using HipLock = std::lock_guard<std::recursive_mutex>;
 // columns and ty rows; ty = 0 picks the smallest ty for which every
 // wavefront of the grid is resident at once (one round, no tail).  march = 2:
 // the persistent tile kernel (tx x ty tiles, each block a run of tiles sized
-// so that all blocks are resident at once).
-struct TileShape { int march, tx, ty; };
-constexpr TileShape kTiles[] = {{0, 64, 4}, {0, 32, 8}, {0, 64, 8}, {0, 128, 4}, {0, 32, 16},
-                                {1, 62, 0}, {1, 62, 8}, {1, 62, 16}, {1, 62, 4},
-                                {2, 64, 4}, {2, 32, 8}};
+// so that all blocks are resident at once).  late = 1: the LDS tile kernel
+// issuing the outputs' AB-history loads after the halo stage (faster when the
+// grid is only a few rounds of resident blocks).
+struct TileShape { int march, tx, ty, late; };
+constexpr TileShape kTiles[] = {{0, 64, 4, 0}, {0, 32, 8, 0}, {0, 64, 8, 0}, {0, 128, 4, 0}, {0, 32, 16, 0},
+                                {1, 62, 0, 0}, {1, 62, 8, 0}, {1, 62, 16, 0}, {1, 62, 4, 0},
+                                {2, 64, 4, 0}, {2, 32, 8, 0}, {0, 32, 8, 1}, {0, 64, 4, 1}};
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 constexpr int kMaxTracers = 2;
 
@@ -590,13 +592,13 @@ Dyn<T> make_dyn(swm_ctx* c, int tile) {
     return d;
 }
 
-template <typename T, int TX, int TY, bool DIAG>
+template <typename T, int TX, int TY, bool DIAG, bool LATE = false>
 void launch_tile(swm_ctx* c, const Dyn<T>& d) {
     const int var = c->d.variant;
     const int nt = (int)c->trc.size();
     dim3 grid(d.nblk), block(256);
 #define SWM_LAUNCH(V, N) \
-    hipLaunchKernelGGL((swm::step_kernel<T, V, N, TX, TY, DIAG>), grid, block, 0, c->stream, d)
+    hipLaunchKernelGGL((swm::step_kernel<T, V, N, TX, TY, DIAG, LATE>), grid, block, 0, c->stream, d)
     if (var == SWM_LINEARISED_MEAN_STATE) {   // no tracers on the linear variants
         SWM_LAUNCH(swm::VAR_MS, 0);
     } else if (var == SWM_FULLY_NONLINEAR) {
@@ -660,6 +662,8 @@ int launch_step_t(swm_ctx* c, const Dyn<T>& d, int tile) {
         case 5: case 6: case 7: case 8: launch_march<T>(c, d); break;
         case 9: launch_persist<T, 64, 4>(c, d); break;
         case 10: launch_persist<T, 32, 8>(c, d); break;
+        case 11: launch_tile<T, 32, 8, false, true>(c, d); break;
+        case 12: launch_tile<T, 64, 4, false, true>(c, d); break;
         default: launch_tile<T, 64, 4, false>(c, d); break;
         }
     }
@@ -872,8 +876,10 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
     // Default shape from measured step times on MI355X: 32x8 tiles carry less
     // halo per output cell and win in FP32 at every size and in FP64 up to
     // about 3072^2 cells; past that the FP64 kernel is HBM-bound and the
-    // longer 512-byte rows of 64x4 tiles win.
-    c->tile = (c->fp32 || (long long)d->Nx * d->Ny < 3072LL * 3072LL) ? 1 : 0;
+    // longer 512-byte rows of 64x4 tiles win.  Grids of up to 2^21 cells are
+    // only a few rounds of resident blocks: there the late-history order wins.
+    const long long cells = (long long)d->Nx * d->Ny;
+    c->tile = (c->fp32 || cells < 3072LL * 3072LL) ? (cells <= (1LL << 21) ? 11 : 1) : 0;
     if (const char* e = getenv("SWM_GPU_TILE")) c->tile = std::max(0, std::min(kNumTiles - 1, atoi(e)));
     if (d->variant == SWM_LINEARISED_MEAN_STATE && kTiles[c->tile].march) c->tile = 1;   // LDS tile kernels only
     if (const char* e = getenv("SWM_GPU_GRAPH")) c->use_graph = atoi(e) != 0;
@@ -1392,7 +1398,7 @@ int swm_gpu_set_kernel_shape(swm_ctx* c, int32_t shape) {
     if (!c) return 1;
     if (shape < 0 || shape >= kNumTiles) return fail(c, "no such kernel shape", 1);
     if (c->d.variant == SWM_LINEARISED_MEAN_STATE && kTiles[shape].march)
-        return fail(c, "LINEARISED_MEAN_STATE runs on the LDS tile kernels (shapes 0-4)", 4);
+        return fail(c, "LINEARISED_MEAN_STATE runs on the LDS tile kernels (shapes 0-4, 11, 12)", 4);
     HipLock lock(g_hip_mu);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     drop_graphs(c);

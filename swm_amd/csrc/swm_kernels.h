@@ -195,17 +195,31 @@ __device__ __forceinline__ T trc_adv_r(bool neg, int nummask, int den, const T* 
 // LDS of one tile: the state with a one-cell halo, the derived fields and the
 // per-row coefficient tables of its rows.  eta shares its array with Pot (each
 // cell reads its own eta before writing its Pot), and the mass fluxes MU, MV
-// are recomputed from D, u, v where they are needed, which keeps the FP64
+// are formed from Du, Dv and u, v where they are needed, which keeps the FP64
 // tile under 32 KiB (five blocks per CU).  NTR is the largest tracer count the
 // kernel handles.
-template <typename T, int NTR, int E, int EH, int ZE = 1>
+template <typename T, int NTR, int E, int EH, int ZE = 1, int DUE = 1, int DVE = 1>
 struct TileLds {
     T u[E], v[E], D[E], Pot[E], Ed[E], pll[E], plt[E];   // Pot holds eta until phase 2 replaces it
     T zt[ZE], et[ZE];    // LINEARISED_MEAN_STATE (ZE = E): relative vorticity and eta for phase 3
+    T Du[DUE], Dv[DVE];  // interp2(D, eta2u/eta2v_noland) of every halo cell (when DUE / DVE = E)
     T CH[NTR][E], C[NTR][E];
     // per-row tables of the EH tile rows (cu also of the row above them)
     T za[EH][2], cu[EH + 1], cH[EH], fH[EH], pltc[EH][12], pllc[EH][12];
     uint8_t m[E];
+};
+
+// Tiles that keep Du and Dv in LDS, computed once per halo cell in phase 2
+// instead of at each stencil point in phase 3: the FP32 tiles, which have
+// LDS to spare at their occupancy (measured 0.8 % faster at 1024^2).  FP64
+// recomputes them (no faster with Du kept, and the 64-wide tiles would drop
+// to four blocks per CU).
+template <typename T, int TX, int TY>
+struct LdsDuDv {
+    static constexpr int E = (TX + 2) * (TY + 2);
+    static constexpr bool SDU = sizeof(T) == 4;
+    static constexpr bool SDV = sizeof(T) == 4;
+    static constexpr int DUE = SDU ? E : 1, DVE = SDV ? E : 1;
 };
 
 // Global loads and stores through an address-space-1 pointer, and the statics
@@ -276,9 +290,57 @@ struct TileShapeC {
     static constexpr int NO = (TX * TY + NT - 1) / NT, NE = (E + NT - 1) / NT;
 };
 
+// Issue the loads of the outputs' AB history (2 slots per field and tracer)
+// and eta at the outputs into R.
+template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, typename RG>
+__device__ __forceinline__ void tile_load_hist(const Dyn<T>& d, const int t, RG& R) {
+    if (DIAG) return;
+    constexpr int NT = 256;
+    constexpr int NO = (TX * TY + NT - 1) / NT;
+    CStatics<T>& a = *(CStatics<T>*)d.s;
+    const int tid = threadIdx.x;
+    const int i0 = (t % d.ntx) * TX;
+    const int j0 = (t / d.ntx) * TY;
+    const int Nx = a.Nx, Ny = a.Ny, P = a.P;
+    T* const st = d.state;
+    const long long pl = d.plane;
+    const int r0 = d.ring, r1 = d.ring == 2 ? 0 : d.ring + 1;
+    const T* const se = st + (PL_E + d.cur) * pl;
+    const T* const Gu1 = st + (PL_GU + r0) * pl;
+    const T* const Gu2 = st + (PL_GU + r1) * pl;
+    const T* const Gv1 = st + (PL_GV + r0) * pl;
+    const T* const Gv2 = st + (PL_GV + r1) * pl;
+    const T* const Ge1 = st + (PL_GE + r0) * pl;
+    const T* const Ge2 = st + (PL_GE + r1) * pl;
+    auto& pg = R.g;
+    auto& pt = R.gt;
+    {
+
+#pragma unroll
+        for (int q = 0; q < NO; ++q) {
+            const int o = tid + q * NT;
+            const int i = i0 + o % TX, j = j0 + o / TX;
+            if (o < TX * TY && i < Nx && j < Ny) {
+                const size_t off = (size_t)j * P + i;
+                pg[q][0] = gld(Gu1, off); pg[q][1] = gld(Gu2, off);
+                pg[q][2] = gld(Gv1, off); pg[q][3] = gld(Gv2, off);
+                pg[q][4] = gld(Ge1, off); pg[q][5] = gld(Ge2, off);
+                R.eo[q] = gld(se, off);
+#pragma unroll
+                for (int n = 0; n < NTRC; ++n) {
+                    const T* tb = st + (PL_TRC + PL_PER_TRC * n + 2) * pl;
+                    pt[q][n][0] = gld(tb, r0 * pl + off);
+                    pt[q][n][1] = gld(tb, r1 * pl + off);
+                }
+            }
+        }
+    }
+
+}
+
 // Issue every global load of tile t: the state with its halo, the per-row
 // tables and the AB history of the outputs.
-template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, typename RG>
+template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, bool HIST = true, typename RG>
 __device__ __forceinline__ void tile_load(const Dyn<T>& d, const int t, RG& R) {
     constexpr int NT = 256;
     constexpr int EW = TX + 2;
@@ -306,29 +368,7 @@ __device__ __forceinline__ void tile_load(const Dyn<T>& d, const int t, RG& R) {
     const T* const Ge1 = st + (PL_GE + r0) * pl;
     const T* const Ge2 = st + (PL_GE + r1) * pl;
 
-    // ---- prefetch this thread's AB history (independent of the LDS phases)
-    auto& pg = R.g;
-    auto& pt = R.gt;
-    if (!DIAG) {
-#pragma unroll
-        for (int q = 0; q < NO; ++q) {
-            const int o = tid + q * NT;
-            const int i = i0 + o % TX, j = j0 + o / TX;
-            if (o < TX * TY && i < Nx && j < Ny) {
-                const size_t off = (size_t)j * P + i;
-                pg[q][0] = gld(Gu1, off); pg[q][1] = gld(Gu2, off);
-                pg[q][2] = gld(Gv1, off); pg[q][3] = gld(Gv2, off);
-                pg[q][4] = gld(Ge1, off); pg[q][5] = gld(Ge2, off);
-                R.eo[q] = gld(se, off);
-#pragma unroll
-                for (int n = 0; n < NTRC; ++n) {
-                    const T* tb = st + (PL_TRC + PL_PER_TRC * n + 2) * pl;
-                    pt[q][n][0] = gld(tb, r0 * pl + off);
-                    pt[q][n][1] = gld(tb, r1 * pl + off);
-                }
-            }
-        }
-    }
+    if (HIST) tile_load_hist<T, VAR, NTRC, TX, TY, DIAG>(d, t, R);
 
     // ---- phase 1: state with a one-cell halo (periodic indices) into LDS.
     // Every global load of the block is issued before the first LDS store, so
@@ -580,6 +620,11 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             s_plt[e] = plt;
             if (interior) { gst(a.diag[1], doff, Dh); gst(a.diag[4], doff, zeta); gst(a.diag[5], doff, Pot); }
         }
+        // computeD: Du, Dv kept for phase 3 (Du from column 1, Dv from row 1 on)
+        if constexpr (LdsDuDv<T, TX, TY>::SDU)
+            if (lx >= 1) lds.Du[e] = interp_noland2(s_D[e], s_D[e - 1], OE(m), OE(s_m[e - 1]));
+        if constexpr (LdsDuDv<T, TX, TY>::SDV)
+            if (ly >= 1) lds.Dv[e] = interp_noland2(s_D[e], s_D[e - EW], OE(m), OE(s_m[e - EW]));
         if (interior) {   // computeMassFluxes: MU = Du * u on ocean u, MV = Dv * v on ocean v
             const T Du = interp_noland2(s_D[e], s_D[e - 1], OE(m), OE(s_m[e - 1]));
             const T Dv = interp_noland2(s_D[e], s_D[e - EW], OE(m), OE(s_m[e - EW]));
@@ -658,19 +703,21 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
         const int jp = j + 1 == Ny ? 0 : j + 1;
         const uint8_t m = s_m[k];
         const int oe = OE(m), ou = OU(m), ov = OV(m), oh = OH(m);
-        const T Du = interp_noland2(s_D[k], s_D[kw], oe, OE(s_m[kw]));
-        const T Dv = interp_noland2(s_D[k], s_D[ks], oe, OE(s_m[ks]));
+        constexpr bool SDU = LdsDuDv<T, TX, TY>::SDU, SDV = LdsDuDv<T, TX, TY>::SDV;
+        // Du, Dv at a halo cell: from LDS, or interp2(D, eta2u/eta2v_noland) again
+        auto DUat = [&](int kk) -> T {
+            if constexpr (SDU) return lds.Du[kk];
+            else return interp_noland2(s_D[kk], s_D[kk - 1], OE(s_m[kk]), OE(s_m[kk - 1]));
+        };
+        auto DVat = [&](int kk) -> T {
+            if constexpr (SDV) return lds.Dv[kk];
+            else return interp_noland2(s_D[kk], s_D[kk - EW], OE(s_m[kk]), OE(s_m[kk - EW]));
+        };
+        const T Du = DUat(k);
+        const T Dv = DVat(k);
         // computeMassFluxes at the points the stencils need
-        auto MUat = [&](int kk) -> T {
-            const int mk = s_m[kk];
-            const T Du_ = interp_noland2(s_D[kk], s_D[kk - 1], OE(mk), OE(s_m[kk - 1]));
-            return OU(mk) ? Du_ * s_u[kk] : T(0);
-        };
-        auto MVat = [&](int kk) -> T {
-            const int mk = s_m[kk];
-            const T Dv_ = interp_noland2(s_D[kk], s_D[kk - EW], OE(mk), OE(s_m[kk - EW]));
-            return OV(mk) ? Dv_ * s_v[kk] : T(0);
-        };
+        auto MUat = [&](int kk) -> T { return OU(s_m[kk]) ? DUat(kk) * s_u[kk] : T(0); };
+        auto MVat = [&](int kk) -> T { return OV(s_m[kk]) ? DVat(kk) * s_v[kk] : T(0); };
         const T MUk = ou ? Du * s_u[k] : T(0), MVk = ov ? Dv * s_v[k] : T(0);
         const T MUe = MUat(ke), MUs = MUat(ks), MUse = MUat(kse);
         const T MVn = MVat(kn), MVw = MVat(kw), MVnw = MVat(knw);
@@ -841,8 +888,8 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             T diff = T(0);
             if (oe) {
                 const T kap = a.kappa[n];
-                const T hue = interp_noland2(s_D[ke], s_D[k], oee, oe);
-                const T hvn = interp_noland2(s_D[kn], s_D[k], oen, oe);
+                const T hue = DUat(ke);   // interp2 of D(i+1,j), D(i,j)
+                const T hvn = DVat(kn);   // interp2 of D(i,j+1), D(i,j)
                 const T c9 = oue ? rt(a.t9, j) : T(0.0);
                 const T c10 = ou ? -rt(a.t9, j) : T(-0.0);
                 const T c11 = ovn ? rt(a.t11, j) : T(0.0);
@@ -881,22 +928,31 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
 
 // One 256-thread block advances tile t (TX x TY cells) of the domain in d by
 // one step.  L is a TileLds with room for at least NTRC tracers.
-template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, typename L>
+// LATE: issue the outputs' AB-history loads after the halo is in LDS instead
+// of with the halo loads.  When every block of the grid starts at once (a few
+// rounds of resident blocks, as at 1024^2) all blocks load together and then
+// compute together; moving a third of the loads behind the halo stage spreads
+// that burst over the barrier and phase 2 (-6 % at 1024^2).  With many rounds
+// the blocks drift apart and the early prefetch hides more latency (+3 % at
+// 2048^2 for the late order).
+template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, bool LATE = false, typename L>
 __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) {
     using S = TileShapeC<TX, TY>;
     TileRegs<T, S::NE, S::NO, (NTRC > 0 ? NTRC : 1)> R;
-    tile_load<T, VAR, NTRC, TX, TY, DIAG>(d, t, R);
+    tile_load<T, VAR, NTRC, TX, TY, DIAG, !LATE>(d, t, R);
     tile_stage<T, VAR, NTRC, TX, TY, DIAG>(d, t, R, lds);
+    if (LATE) tile_load_hist<T, VAR, NTRC, TX, TY, DIAG>(d, t, R);
     __syncthreads();
     tile_compute<T, VAR, NTRC, TX, TY, DIAG>(d, t, lds, R.g, R.gt, R.eo);
 }
 
-template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG>
+template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, bool LATE = false>
 __global__ void __launch_bounds__(256)
 step_kernel(const Dyn<T> d) {
+    using DD = LdsDuDv<T, TX, TY>;
     __shared__ TileLds<T, (NTRC > 0 ? NTRC : 1), (TX + 2) * (TY + 2), TY + 2,
-                       (VAR == VAR_MS ? (TX + 2) * (TY + 2) : 1)> lds;
-    tile_body<T, VAR, NTRC, TX, TY, DIAG>(d, xcd_tile(blockIdx.x, d.nblk), lds);
+                       (VAR == VAR_MS ? (TX + 2) * (TY + 2) : 1), DD::DUE, DD::DVE> lds;
+    tile_body<T, VAR, NTRC, TX, TY, DIAG, LATE>(d, xcd_tile(blockIdx.x, d.nblk), lds);
 }
 
 // Persistent tile kernel: block b advances the tiles [b*tpb, (b+1)*tpb) in
@@ -908,7 +964,8 @@ __global__ void __launch_bounds__(256)
 step_kernel_p(const Dyn<T> d) {
     using S = TileShapeC<TX, TY>;
     constexpr int NTR = NTRC > 0 ? NTRC : 1;
-    __shared__ TileLds<T, NTR, S::E, S::EH> lds;
+    using DD = LdsDuDv<T, TX, TY>;
+    __shared__ TileLds<T, NTR, S::E, S::EH, 1, DD::DUE, DD::DVE> lds;
     const int b = xcd_tile(blockIdx.x, gridDim.x);
     const int t0 = b * d.tpb;
     const int t1 = t0 + d.tpb < d.nblk ? t0 + d.tpb : d.nblk;
@@ -962,7 +1019,8 @@ template <typename T, int VAR, int NTRC_MAX, int TX, int TY>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 batch_kernel(const Member<T>* __restrict__ mem, const MemberStep* __restrict__ stp, const int* __restrict__ owner,
              int nwork) {
-    __shared__ TileLds<T, (NTRC_MAX > 0 ? NTRC_MAX : 1), (TX + 2) * (TY + 2), TY + 2> lds;
+    using DD = LdsDuDv<T, TX, TY>;
+    __shared__ TileLds<T, (NTRC_MAX > 0 ? NTRC_MAX : 1), (TX + 2) * (TY + 2), TY + 2, 1, DD::DUE, DD::DVE> lds;
     const int w = xcd_tile(blockIdx.x, nwork);
     const int lo = owner[w];   // member whose work range holds w
     const Member<T>& m = mem[lo];

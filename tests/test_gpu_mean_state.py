@@ -39,7 +39,7 @@ def run_pair(s, checkpoints, names, shape=None):
     o.close()
 
 
-@pytest.mark.parametrize("shape", [1, 0, 2, 3, 4])
+@pytest.mark.parametrize("shape", [11, 1, 0, 2, 3, 4, 12])
 def test_mean_state_every_tile_shape(gpu_lib, shape):
     s = configs.mean_state_basin(seed=4, Nx=100, Ny=70)
     run_pair(s, [1, 2, 3, 9, 31], STATE + DERIVED, shape=shape)
@@ -97,5 +97,5 @@ def test_mean_state_refusals(gpu_lib):
         with pytest.raises(gpu.SwmGpuError):
             b.attach(g, 3)
     best, times = g.tune(steps=3)
-    assert 0 <= best <= 4 and all(t < 0 for t in times[5:])
+    assert best in (0, 1, 2, 3, 4, 11, 12) and all(t < 0 for t in times[5:11]) and min(times[11:]) > 0
     g.close()

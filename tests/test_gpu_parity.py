@@ -241,7 +241,7 @@ def test_every_kernel_shape_bit_identical(gpu_lib, monkeypatch):
     host.finalize(s)
     o = oracle.Oracle(s)
     o.step(9)
-    for tile in range(11):
+    for tile in range(gpu.load_library().swm_gpu_kernel_shape_count()):
         monkeypatch.setenv("SWM_GPU_TILE", str(tile))
         g = gpu.SwmGpu(s).init_from_setup()
         g.step(9)
