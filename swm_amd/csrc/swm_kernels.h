@@ -254,7 +254,11 @@ template <typename T, typename S>
 __device__ __forceinline__ void apply_streams(S& a, const Dyn<T>& d, size_t off, int ou, int ov, int oe, T Du, T Dv,
                                               T& Fx, T& Fy, T& Fe) {
     const int ns = a.nstream;
-    for (int n = 0; n < ns; ++n) {
+    // unrolled over the fixed maximum: every d.ss[n] index is a constant, so a
+    // Dyn built in registers (the batched kernel) is not demoted to scratch
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        if (n >= ns) break;
         const int comp = a.scomp[n];
         if (!(comp == 0 ? ou : comp == 1 ? ov : oe)) continue;
         const T* r = a.srec[n];
