@@ -207,6 +207,7 @@ struct swm_ctx {
     double* norm_buf = nullptr;  // ||B|| partials and result of the device-resident SOR
     bool chi_valid[2] = {};      // sor_buf[w][0] holds a solution to warm-start from
     int sor_every = 1;           // fixed-omega mode: convergence test every sor_every sweeps
+    bool sor_blocked = true;     // fixed-omega mode: several sweeps per launch when sor_every allows
     struct Avg {                 // averaging buffer of a diag task
         int field;               // 0 u, 1 v, 2 eta, 3 CH of tracer `tracer`
         int tracer;
@@ -235,6 +236,7 @@ struct swm_ctx {
     long long* sor_iters = nullptr;
     double* sor_norm = nullptr;
     void* sor_buf[2][2] = {};    // [fp32][chi|B]
+    void* sor_alt[2] = {};       // second chi buffer of the blocked sweeps (ping-pong)
     void* sor_ftab[2][4] = {};   // coefficient tables in the sweep precision
     void* dza = nullptr;
     swm_batch* batch = nullptr;  // batch this context is attached to
@@ -883,6 +885,7 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
     if (const char* e = getenv("SWM_GPU_TILE")) c->tile = std::max(0, std::min(kNumTiles - 1, atoi(e)));
     if (d->variant == SWM_LINEARISED_MEAN_STATE && kTiles[c->tile].march) c->tile = 1;   // LDS tile kernels only
     if (const char* e = getenv("SWM_GPU_GRAPH")) c->use_graph = atoi(e) != 0;
+    if (const char* e = getenv("SWM_GPU_SOR_BLOCK")) c->sor_blocked = atoi(e) != 0;
     const int align = c->fp32 ? 32 : 16;   // 128-byte aligned rows
     c->P = (c->Nx + align - 1) / align * align;
     c->plane = (long long)c->P * c->Ny;
@@ -1186,7 +1189,10 @@ namespace {
 // the sweep precision, coefficient tables in that precision, the omega schedule.
 int sor_prepare(swm_ctx* c, bool f32, int nblk) {
     if (!c->sor_partial) {
-        c->sor_partial = (double*)dalloc(c, sizeof(double) * 2 * (size_t)nblk);
+        // partial sums of the colour launches (2 x nblk) or of the blocked
+        // sweeps (one per 64 x 16 tile)
+        const size_t nb = std::max<size_t>(2 * (size_t)nblk, (size_t)((c->Nx + 63) / 64) * ((c->Ny + 15) / 16));
+        c->sor_partial = (double*)dalloc(c, sizeof(double) * nb);
         c->sor_done = (int*)dalloc(c, sizeof(int));
         c->sor_iters = (long long*)dalloc(c, sizeof(long long));
         c->sor_norm = (double*)dalloc(c, sizeof(double));
@@ -1198,7 +1204,8 @@ int sor_prepare(swm_ctx* c, bool f32, int nblk) {
     if (!c->sor_buf[w][0]) {
         c->sor_buf[w][0] = dalloc(c, bytes);
         c->sor_buf[w][1] = dalloc(c, bytes);
-        if (!c->sor_buf[w][0] || !c->sor_buf[w][1]) return fail(c, "out of device memory", 3);
+        c->sor_alt[w] = dalloc(c, bytes);
+        if (!c->sor_buf[w][0] || !c->sor_buf[w][1] || !c->sor_alt[w]) return fail(c, "out of device memory", 3);
     }
     if (!c->sor_ftab[w][0]) {
         const int Ny = c->Ny;
@@ -1233,6 +1240,30 @@ int sor_prepare(swm_ctx* c, bool f32, int nblk) {
     return 0;
 }
 
+// Fixed-omega sweeps l .. l+K-1 in one blocked launch (sor_block) from chi_in
+// into chi_out, then the convergence test if sweep l+K-1 is a test sweep.
+template <typename T, int K>
+void sor_block_launch(swm_ctx* c, const void* chi_in, void* chi_out, void* dB, void* const* tabs, double omega,
+                      double tol, double bnorm, int64_t l, bool check) {
+    constexpr int TX = 64, TY = 16;
+    const int ntx = (c->Nx + TX - 1) / TX;
+    const int nb = ntx * ((c->Ny + TY - 1) / TY);
+    swm::SorArgs<T> a{};
+    a.Nx = c->Nx; a.Ny = c->Ny; a.P = c->P; a.color = 0; a.sweep = (int)l;
+    a.chi = nullptr; a.B = (const T*)dB; a.mask = c->mask;
+    a.relax = nullptr; a.nrelax = 0; a.omega = omega;
+    a.c1t = (const T*)tabs[0]; a.c3t = (const T*)tabs[1]; a.c4t = (const T*)tabs[2]; a.c5t = (const T*)tabs[3];
+    a.partial = c->sor_partial;
+    a.done = c->sor_done;
+    a.residual_mode = 1;
+    a.accumulate = check ? 1 : 0;
+    hipLaunchKernelGGL((swm::sor_block<T, TX, TY, K>), dim3(nb), dim3(256), 0, c->stream, a, (const T*)chi_in,
+                       (T*)chi_out, ntx);
+    if (check)
+        hipLaunchKernelGGL(swm::sor_check, dim3(1), dim3(256), 0, c->stream, (const double*)c->sor_partial, nb,
+                           (int)(l + K - 1), tol, bnorm, 1, c->sor_done, c->sor_iters, c->sor_norm);
+}
+
 template <typename T>
 int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mode, double omega, double tol,
                       double bnorm, int64_t max_sweeps, int nblk, int* host_done) {
@@ -1240,6 +1271,31 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
     const int batch = std::max(32, 4 * every);
     *host_done = 0;
     int64_t l = 1;
+    // Fixed-omega mode testing every 2 or 4 sweeps (or a multiple): K sweeps
+    // per blocked launch, chi alternating between dchi and the second buffer.
+    const int K = (ref_mode || !c->sor_blocked) ? 0 : (every % 4 == 0 ? 4 : (every % 2 == 0 ? 2 : 0));
+    if (K) {
+        const int w = std::is_same<T, float>::value ? 1 : 0;
+        void* cur = dchi;
+        void* alt = c->sor_alt[w];
+        const int64_t nfull = max_sweeps / K * K;
+        while (l + K - 1 <= nfull && !*host_done) {
+            const int64_t lend = std::min<int64_t>(nfull, l + batch - 1);
+            for (; l + K - 1 <= lend; l += K) {
+                const bool check = ((l + K - 1) % every) == 0 || l + K - 1 == max_sweeps;
+                if (K == 4) sor_block_launch<T, 4>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check);
+                else sor_block_launch<T, 2>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check);
+                std::swap(cur, alt);
+            }
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipMemcpyAsync(host_done, c->sor_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        if (cur != dchi)   // the result in the caller's buffer
+            HIPCHK(c, hipMemcpyAsync(dchi, cur, (size_t)c->P * c->Ny * sizeof(T), hipMemcpyDeviceToDevice,
+                                     c->stream));
+        if (*host_done) return 0;
+    }
     while (l <= max_sweeps && !*host_done) {
         const int64_t lend = std::min<int64_t>(max_sweeps, l + batch - 1);
         for (; l <= lend; ++l) {

@@ -1516,6 +1516,82 @@ __global__ void __launch_bounds__(256) sor_sweep(const SorArgs<T> a) {
     if (threadIdx.x == 0) a.partial[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
+// K fixed-omega sweeps in one launch on a TX x TY tile held in LDS with a
+// 2K-cell halo (read from chi_in, written to chi_out: blocks read their halos
+// while neighbours write).  Half sweep hs updates the colour-(hs % 2) cells at
+// least hs + 1 cells inside the staged block, whose dependency cones stay
+// inside it, so the K sweeps give each output cell exactly the values of K
+// global red-black sweeps (sor_sweep, same arithmetic).  With accumulate the
+// residuals r of the last sweep's updates of the output cells are summed in
+// FP64 in a fixed order into partial[block].
+template <typename T, int TX, int TY, int K>
+__global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __restrict__ chi_in,
+                                                 T* __restrict__ chi_out, int ntx) {
+    constexpr int H = 2 * K, W = TX + 2 * H, HH = TY + 2 * H, E = W * HH;
+    __shared__ T s_chi[E];
+    __shared__ T s_B[E];
+    __shared__ uint8_t s_m[E];
+    __shared__ double red[256 / 64];
+    const int Nx = a.Nx, Ny = a.Ny, P = a.P;
+    const int i0 = (blockIdx.x % ntx) * TX, j0 = (blockIdx.x / ntx) * TY;
+    const int tid = threadIdx.x;
+    if (*a.done) {   // converged: carry the output cells over to the other buffer
+        for (int o = tid; o < TX * TY; o += 256) {
+            const int i = i0 + o % TX, j = j0 + o / TX;
+            if (i < Nx && j < Ny) chi_out[(size_t)j * P + i] = chi_in[(size_t)j * P + i];
+        }
+        return;
+    }
+    for (int e = tid; e < E; e += 256) {
+        const int lx = e % W, ly = e / W;
+        const size_t off = (size_t)wrap(j0 - H + ly, Ny) * P + wrap(i0 - H + lx, Nx);
+        s_chi[e] = chi_in[off];
+        s_B[e] = a.B[off];
+        s_m[e] = a.mask[off];
+    }
+    __syncthreads();
+    double acc = 0.0;
+#pragma unroll 1
+    for (int hs = 0; hs < 2 * K; ++hs) {
+        const int color = hs & 1;
+        const int lo = hs + 1, rw = W - 2 * lo, rh = HH - 2 * lo;
+        const bool tail = a.accumulate && hs >= 2 * K - 2;
+        for (int q = tid; q < rw * rh; q += 256) {
+            const int lx = lo + q % rw, ly = lo + q / rw;
+            const int gi = wrap(i0 - H + lx, Nx), j = wrap(j0 - H + ly, Ny);
+            if (((gi + j) & 1) != color) continue;
+            const int e = ly * W + lx;
+            const uint8_t m = s_m[e];
+            if (!(m & OC_E)) continue;
+            const uint8_t mip = s_m[e + 1], mjp = s_m[e + W];
+            const int ouip = (mip >> 1) & 1, ou = (m >> 1) & 1, ovjp = (mjp >> 2) & 1, ov = (m >> 2) & 1;
+            const T c1 = ouip ? a.c1t[j] : T(0);
+            const T c2 = ou ? a.c1t[j] : T(0);
+            const T c3 = ovjp ? a.c3t[j] : T(0);
+            const T c4 = ov ? a.c4t[j] : T(0);
+            const T c5 = a.c5t[((j * 3 + ouip + ou) * 2 + ovjp) * 2 + ov];
+            const T x = s_chi[e];
+            const T lhs = (((c1 * s_chi[e + 1] + c2 * s_chi[e - 1]) + c3 * s_chi[e + W]) + c4 * s_chi[e - W]) + c5 * x;
+            const T r = lhs - s_B[e];
+            const T res = T(a.omega) * r / c5;
+            s_chi[e] = x - res;
+            if (tail && lx >= H && lx < H + TX && ly >= H && ly < H + TY && i0 + lx - H < Nx && j0 + ly - H < Ny)
+                acc += (double)r * (double)r;
+        }
+        __syncthreads();
+    }
+    for (int o = tid; o < TX * TY; o += 256) {
+        const int i = i0 + o % TX, j = j0 + o / TX;
+        if (i < Nx && j < Ny) chi_out[(size_t)j * P + i] = s_chi[(H + o / TX) * W + H + o % TX];
+    }
+    if (!a.accumulate) return;   // uniform
+    for (int s = 32; s >= 1; s >>= 1) acc += __shfl_xor(acc, s, 64);
+    const int lane = tid & 63, wv = tid >> 6;
+    if (lane == 0) red[wv] = acc;
+    __syncthreads();
+    if (tid == 0) a.partial[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
 // Sum the block partials in a fixed order (one block), test convergence.
 // mode 0: sqrt(sum) < eps (reference); mode 1: sqrt(sum)/bnorm < eps.
 __global__ void __launch_bounds__(256)

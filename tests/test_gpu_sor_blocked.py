@@ -1,0 +1,80 @@
+"""Blocked fixed-omega SOR (several red-black sweeps per launch on LDS tiles,
+used when the convergence test runs every 2 or 4 sweeps or a multiple): the
+same chi, bit for bit, as the oracle's sequential sweeps in FP64 and as the
+one-colour-per-launch sweeps in FP32, for sweep counts that leave a remainder,
+grids that no tile divides and a grid narrower than the tile's halo."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import same
+from swm_amd import configs, gpu, host
+
+pytestmark = pytest.mark.gpu
+
+
+def basin(nx, ny, land, seed):
+    s = configs.basin(nx, ny, 10e3, 10e3, variant=host.FULLY_NONLINEAR, H0=500.0, dt=10.0, seed=seed,
+                      land_fraction=land)
+    return host.finalize(s)
+
+
+def rhs(o, s, seed):
+    rng = np.random.default_rng(seed)
+    return o.divergence(rng.standard_normal(s.shape) * s.grids["u"].ocean,
+                        rng.standard_normal(s.shape) * s.grids["v"].ocean)
+
+
+@pytest.mark.parametrize("nx,ny,every,sweeps", [(96, 64, 4, 40), (130, 70, 4, 42), (130, 70, 2, 41),
+                                                (7, 5, 4, 12), (200, 33, 16, 50)])
+def test_blocked_sweeps_equal_sequential_sweeps(gpu_lib, nx, ny, every, sweeps):
+    s = basin(nx, ny, 0.08 if nx > 20 else 0.0, seed=nx + ny)
+    o = oracle.Oracle(s)
+    B = rhs(o, s, 3)
+    chi0 = 1e3 * np.random.default_rng(5).standard_normal(s.shape)
+    chi_o, it_o, _, rc_o = o.sor_fixed(B, chi0, 1.7, 0.0, sweeps)     # tol 0: exactly `sweeps` sweeps
+    g = gpu.SwmGpu(s)
+    g.set_sor_check_every(every)
+    chi_g, it_g, _, ok = g.sor_fixed(B, chi0, 1.7, 0.0, sweeps)
+    assert it_o == it_g == sweeps and not ok and rc_o == 1
+    assert same(chi_g, chi_o)
+    g.close()
+    o.close()
+
+
+def test_blocked_fp32_sweeps_equal_unblocked(gpu_lib, monkeypatch):
+    s = basin(150, 90, 0.05, seed=9)
+    o = oracle.Oracle(s)
+    B = rhs(o, s, 4)
+    o.close()
+    out = {}
+    for blocked in ("1", "0"):
+        monkeypatch.setenv("SWM_GPU_SOR_BLOCK", blocked)
+        g = gpu.SwmGpu(s)
+        g.set_sor_check_every(4)
+        out[blocked] = g.sor_fixed(B, np.zeros(s.shape), 1.9, 0.0, 36, fp32_sweeps=True)
+        g.close()
+    assert out["1"][1] == out["0"][1] == 36
+    assert same(out["1"][0], out["0"][0])
+
+
+def test_blocked_converges_like_unblocked(gpu_lib, monkeypatch):
+    """With a tolerance the blocked run stops at a test sweep like the
+    unblocked run with the same cadence (the residual sums differ only in
+    summation order)."""
+    s = basin(96, 64, 0.05, seed=2)
+    o = oracle.Oracle(s)
+    B = rhs(o, s, 6)
+    o.close()
+    res = {}
+    for blocked in ("1", "0"):
+        monkeypatch.setenv("SWM_GPU_SOR_BLOCK", blocked)
+        g = gpu.SwmGpu(s)
+        g.set_sor_check_every(8)
+        res[blocked] = g.sor_fixed(B, np.zeros(s.shape), 1.9, 1e-9, 5000)
+        g.close()
+    chi_b, it_b, rr_b, ok_b = res["1"]
+    chi_u, it_u, rr_u, ok_u = res["0"]
+    assert ok_b and ok_u and it_b % 8 == 0 and abs(it_b - it_u) <= 8 and rr_b < 1e-9
+    if it_b == it_u:
+        assert same(chi_b, chi_u)
