@@ -197,6 +197,7 @@ struct swm_ctx {
     FieldDev fields[F_COUNT];
     void* tab[T_COUNT] = {};
     void* statics = nullptr;     // device Statics<T>
+    void* rowtab = nullptr;      // device [Ny][RT_N] packed per-row values (Statics::rowtab)
     double ab[5][5] = {};
     std::vector<Tracer> trc;
     std::vector<FStream> streams;   // time-interpolated forcing streams, in processing order
@@ -504,9 +505,16 @@ int write_statics_t(swm_ctx* c) {
         a.scomp[n] = c->streams[n].comp;
         a.srec[n] = (const T*)c->streams[n].rec;
     }
+    static_assert(swm::RT_N >= swm::RT_TRC + swm::RT_PER_TRC * kMaxTracers, "rowtab has room for every tracer");
     if (!c->statics && !(c->statics = dalloc(c, sizeof(Statics<T>)))) return fail(c, "out of device memory", 3);
+    if (!c->rowtab && !(c->rowtab = dalloc(c, (size_t)c->Ny * swm::RT_N * sizeof(T)))) return fail(c, "out of device memory", 3);
+    a.rowtab = (const T*)c->rowtab;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(c->statics, &a, sizeof(a), hipMemcpyHostToDevice));
+    const int nrt = c->Ny * swm::RT_N;
+    swm::pack_rowtab<T><<<(nrt + 255) / 256, 256, 0, c->stream>>>((const Statics<T>*)c->statics, (T*)c->rowtab, c->Ny);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 

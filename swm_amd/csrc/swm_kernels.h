@@ -37,6 +37,19 @@ __device__ __forceinline__ T fld(const Fld<T>& f, size_t off, int j, bool ocean)
     return f.full ? f.full[off] : f.rows[2 * j + (ocean ? 0 : 1)];
 }
 
+// Packed per-row values that phase 3 of the tile kernels reads, RT_N per row
+// (Statics::rowtab): built on the device from the per-row tables and the
+// row-compressed fields whenever the statics are written.  A field stored as
+// a full array leaves its two entries 0 and is read from the array.  Tracer n
+// has RT_PER_TRC entries from RT_TRC + n * RT_PER_TRC: timpl, gC, C0 (ocean,
+// land each).
+enum : int {
+    RT_CV = 0, RT_ACE = 1, RT_EDU = 2, RT_LMU = 3, RT_LMV = 9, RT_T9 = 15, RT_T11 = 16, RT_T12 = 17,
+    RT_TA = 18, RT_TB = 20, RT_TC = 22, RT_FEC = 24, RT_FXC = 26, RT_FYC = 28, RT_IMU = 30, RT_IMV = 32,
+    RT_IME = 34, RT_HE = 36, RT_TAUX = 38, RT_TAUY = 40, RT_TRC = 42, RT_PER_TRC = 6,
+    RT_N = RT_TRC + 2 * RT_PER_TRC
+};
+
 // Static inputs of the step kernel, resident in device memory and read through
 // one pointer (scalar loads on use), so the per-launch argument block stays
 // small and the kernel does not spill uniform values out of SGPRs.
@@ -59,6 +72,7 @@ struct Statics {
     const T* pltc;   // [Ny][ou(j)][ou(j-1)][3]
     const T *tA, *tB, *tC;   // [Ny][2] tracer advection magnitudes, denominator 1 / 2
     const T *t9, *t11, *t12; // [Ny] tracer diffusion magnitudes
+    const T* rowtab;         // [Ny][RT_N] packed per-row values of phase 3
     // tracers
     T kappa[2];
     Fld<T> C0[2], gC[2], timpl[2];
@@ -73,6 +87,43 @@ struct Statics {
     T* diag[14];      // D Dh Du Dv zeta Pot EDens MU MV F_x F_y F_eta latmix_u latmix_v
     T* tdiag[2][3];   // C uhc vhc per tracer
 };
+
+// Fill Statics::rowtab ([Ny][RT_N]) from the tables and the row-compressed
+// fields; entries of absent tracers and of full-array fields are 0.
+template <typename T>
+__global__ void __launch_bounds__(256) pack_rowtab(const Statics<T>* __restrict__ sp, T* __restrict__ out, int Ny) {
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= Ny * RT_N) return;
+    const Statics<T>& a = *sp;
+    const int j = x / RT_N, k = x % RT_N;
+    auto rows = [&](const Fld<T>& f, int s) -> T { return f.rows ? f.rows[2 * j + s] : T(0); };
+    T v;
+    if (k == RT_CV) v = a.cv[j];
+    else if (k == RT_ACE) v = a.ace[j];
+    else if (k == RT_EDU) v = a.edu[j];
+    else if (k < RT_LMV) v = a.lmu[6 * j + k - RT_LMU];
+    else if (k < RT_T9) v = a.lmv[6 * j + k - RT_LMV];
+    else if (k == RT_T9) v = a.t9[j];
+    else if (k == RT_T11) v = a.t11[j];
+    else if (k == RT_T12) v = a.t12[j];
+    else if (k < RT_TB) v = a.tA[2 * j + k - RT_TA];
+    else if (k < RT_TC) v = a.tB[2 * j + k - RT_TB];
+    else if (k < RT_FEC) v = a.tC[2 * j + k - RT_TC];
+    else if (k < RT_FXC) v = rows(a.Fec, k - RT_FEC);
+    else if (k < RT_FYC) v = rows(a.Fxc, k - RT_FXC);
+    else if (k < RT_IMU) v = rows(a.Fyc, k - RT_FYC);
+    else if (k < RT_IMV) v = rows(a.imu, k - RT_IMU);
+    else if (k < RT_IME) v = rows(a.imv, k - RT_IMV);
+    else if (k < RT_HE) v = rows(a.ime, k - RT_IME);
+    else if (k < RT_TAUX) v = rows(a.He, k - RT_HE);
+    else if (k < RT_TAUY) v = rows(a.taux, k - RT_TAUX);
+    else if (k < RT_TRC) v = rows(a.tauy, k - RT_TAUY);
+    else {
+        const int n = (k - RT_TRC) / RT_PER_TRC, r = (k - RT_TRC) % RT_PER_TRC;
+        v = rows(r < 2 ? a.timpl[n] : (r < 4 ? a.gC[n] : a.C0[n]), r & 1);
+    }
+    out[x] = v;
+}
 
 // State planes in one allocation: u, v, eta at two levels, three AB slots per
 // field, then per tracer CH (two levels), G_CH (three slots) and the C buffer.
@@ -198,15 +249,32 @@ __device__ __forceinline__ T trc_adv_r(bool neg, int nummask, int den, const T* 
 // are formed from Du, Dv and u, v where they are needed, which keeps the FP64
 // tile under 32 KiB (five blocks per CU).  NTR is the largest tracer count the
 // kernel handles.
-template <typename T, int NTR, int E, int EH, int ZE = 1, int DUE = 1, int DVE = 1>
+template <typename T, int NTR, int E, int EH, int ZE = 1, int DUE = 1, int DVE = 1, int RTR = 1, int RTK_ = 1>
 struct TileLds {
+    static constexpr int RTK = RTK_;
     T u[E], v[E], D[E], Pot[E], Ed[E], pll[E], plt[E];   // Pot holds eta until phase 2 replaces it
     T zt[ZE], et[ZE];    // LINEARISED_MEAN_STATE (ZE = E): relative vorticity and eta for phase 3
     T Du[DUE], Dv[DVE];  // interp2(D, eta2u/eta2v_noland) of every halo cell (when DUE / DVE = E)
     T CH[NTR][E], C[NTR][E];
     // per-row tables of the EH tile rows (cu also of the row above them)
     T za[EH][2], cu[EH + 1], cH[EH], fH[EH], pltc[EH][12], pllc[EH][12];
+    T rt[RTR][RTK_];     // rowtab rows of the output rows and the row after them (RowTabC)
     uint8_t m[E];
+};
+
+// Phase 3's per-row values: tiles whose wavefronts span several rows stage
+// the rowtab entries of their output rows and of the row after them in LDS
+// (one coalesced copy per tile instead of a vector load per value and lane);
+// 64-wide tiles with one output per thread (a row per wavefront) read them
+// with scalar loads.  K = entries a kernel with NTRC tracers reads.
+template <int TX, int TY, int NTRC>
+struct RowTabC {
+    static constexpr int NO = (TX * TY + 255) / 256;
+    static constexpr bool UROW = TX == 64 && NO == 1;
+    static constexpr int K = RT_TRC + RT_PER_TRC * NTRC;
+    static constexpr int R = TY + 1;
+    static constexpr int LR = UROW ? 1 : R, LK = UROW ? 1 : K;
+    static constexpr int NQ = UROW ? 1 : (R * K + 255) / 256;
 };
 
 // Tiles that keep Du and Dv in LDS, computed once per halo cell in phase 2
@@ -278,8 +346,9 @@ __device__ __forceinline__ void apply_streams(S& a, const Dyn<T>& d, size_t off,
 // Registers of one tile between its loads and their LDS stores: the state with
 // halo (NE cells per thread), the per-row tables and the AB history of the
 // thread's NO output cells.
-template <typename T, int NE, int NO, int NTR>
+template <typename T, int NE, int NO, int NTR, int NQ = 1>
 struct TileRegs {
+    T rt[NQ];    // rowtab values for the LDS row table (RowTabC)
     int m[NE];
     T u[NE], v[NE], e[NE], Ho[NE], Hl[NE], ch[NTR][NE], cb[NTR][NE];
     T tz0, tz1, tcu, tcH, tfH, tcu0, tpc;
@@ -431,6 +500,16 @@ __device__ __forceinline__ void tile_load(const Dyn<T>& d, const int t, RG& R) {
         const int gj = wrap(j0 + r - 1, Ny);
         tpc = c < 12 ? gld(a.pltc, gj * 12 + c) : gld(a.pllc, gj * 12 + c - 12);
     }
+    // rowtab rows j0 .. j0+TY for phase 3 (tiles that keep them in LDS)
+    using RTC = RowTabC<TX, TY, NTRC>;
+    if constexpr (!RTC::UROW) {
+#pragma unroll
+        for (int q = 0; q < RTC::NQ; ++q) {
+            const int x = tid + q * NT;
+            if (x < RTC::R * RTC::K)
+                R.rt[q] = gld(a.rowtab, (size_t)wrap(j0 + x / RTC::K, Ny) * RT_N + x % RTC::K);
+        }
+    }
 }
 
 // Derive D and C from the loaded state and write the tile to LDS.
@@ -529,6 +608,14 @@ __device__ __forceinline__ void tile_stage(const Dyn<T>& d, const int t, const R
             if (c < 12) lds.pltc[r][c] = gld(a.pltc, gj * 12 + c);
             else lds.pllc[r][c - 12] = gld(a.pllc, gj * 12 + c - 12);
         }
+    using RTC = RowTabC<TX, TY, NTRC>;
+    if constexpr (!RTC::UROW) {
+#pragma unroll
+        for (int q = 0; q < RTC::NQ; ++q) {
+            const int x = tid + q * NT;
+            if (x < RTC::R * RTC::K) lds.rt[x / RTC::K][x % RTC::K] = R.rt[q];
+        }
+    }
 }
 
 // Phases 2 and 3 of tile t, after its LDS stage and a barrier: derived fields,
@@ -684,27 +771,30 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
         const int i = i0 + lx - 1;
         const int j = UROW ? __builtin_amdgcn_readfirstlane(j0 + ly - 1) : j0 + ly - 1;
         if (o >= TX * TY || i >= Nx || j >= Ny) continue;
-        auto rt = [&](const T* p, int idx) -> T {   // per-row table value
-            if constexpr (UROW) return ldc(p, (size_t)idx);
-            else return gld(p, (size_t)idx);
-        };
-        auto fl = [&](auto& f, size_t at, bool ocean) -> T {   // static field at (i, j)
-            if constexpr (UROW) {
-                if (f.full) return gld(f.full, at);
-                const T fo = ldc(f.rows, 2 * (size_t)j), fl_ = ldc(f.rows, 2 * (size_t)j + 1);
-                return ocean ? fo : fl_;
-            } else {
-                return f.full ? gld(f.full, at) : gld(f.rows, 2 * (size_t)j + (ocean ? 0 : 1));
-            }
-        };
-        auto ta = [&](bool neg, int nummask, int den, const T* tab) -> T {
-            if constexpr (UROW) return trc_adv_r<T>(neg, nummask, den, tab, j);
-            else return trc_adv<T>(neg, nummask, den, tab, j);
-        };
         const size_t off = (size_t)j * P + i;
         const int k = ly * EW + lx;
         const int ke = k + 1, kw = k - 1, kn = k + EW, ks = k - EW, knw = kn - 1, kse = ks + 1;
         const int jp = j + 1 == Ny ? 0 : j + 1;
+        static_assert(RowTabC<TX, TY, NTRC>::UROW == UROW, "row-table layout");
+        auto rv = [&](int e, int dr) -> T {   // rowtab entry e of row j (dr = 0) or jp (dr = 1)
+            if constexpr (UROW) return ldc(a.rowtab, (size_t)(dr ? jp : j) * RT_N + e);
+            else return lds.rt[ly - 1 + dr][e];
+        };
+        // static field at (i, j); e = its rowtab entry, -1 for fields without one
+        auto fl = [&](auto& f, int e, size_t at, bool ocean) -> T {
+            if (f.full) return gld(f.full, at);
+            if (e < 0) return gld(f.rows, 2 * (size_t)j + (ocean ? 0 : 1));
+            const T fo = rv(e, 0), fl_ = rv(e + 1, 0);
+            return ocean ? fo : fl_;
+        };
+        // TRC_coeff magnitude (trc_adv) from the rowtab pair at e
+        auto ta = [&](bool neg, int nummask, int den, int e) -> T {
+            const T v1 = rv(e, 0), v2 = rv(e + 1, 0);
+            if (den == 0) return __builtin_nan("");
+            if (!nummask) return neg ? T(-0.0) : T(0.0);
+            const T v = den == 1 ? v1 : v2;
+            return neg ? -v : v;
+        };
         const uint8_t m = s_m[k];
         const int oe = OE(m), ou = OU(m), ov = OV(m), oh = OH(m);
         constexpr bool SDU = LdsDuDv<T, TX, TY>::SDU, SDV = LdsDuDv<T, TX, TY>::SDV;
@@ -727,20 +817,20 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
         const T MVn = MVat(kn), MVw = MVat(kw), MVnw = MVat(knw);
 
         // SWM_forcing_update (src/swm_forcing_module.f90:135-170)
-        T Fe = fl(a.Fec, off, oe);
-        if (a.has_oe) Fe = Fe + (T(2) * (fl(a.oRe, off, oe) * d.oc[2] - fl(a.oIe, off, oe) * d.os[2])) * T(oe);
-        T Fx = fl(a.Fxc, off, ou);
+        T Fe = fl(a.Fec, RT_FEC, off, oe);
+        if (a.has_oe) Fe = Fe + (T(2) * (fl(a.oRe, -1, off, oe) * d.oc[2] - fl(a.oIe, -1, off, oe) * d.os[2])) * T(oe);
+        T Fx = fl(a.Fxc, RT_FXC, off, ou);
         if (a.has_taux && ou) {
-            const T tx = fl(a.taux, off, true);
+            const T tx = fl(a.taux, RT_TAUX, off, true);
             Fx = Fx + dv(a.has_tau_scale ? a.tau_scale * tx : tx, a.RHO0 * Du);
         }
-        if (a.has_ox) Fx = Fx + (T(2) * (fl(a.oRx, off, ou) * d.oc[0] - fl(a.oIx, off, ou) * d.os[0])) * T(ou);
-        T Fy = fl(a.Fyc, off, ov);
+        if (a.has_ox) Fx = Fx + (T(2) * (fl(a.oRx, -1, off, ou) * d.oc[0] - fl(a.oIx, -1, off, ou) * d.os[0])) * T(ou);
+        T Fy = fl(a.Fyc, RT_FYC, off, ov);
         if (a.has_tauy && ov) {
-            const T ty = fl(a.tauy, off, true);
+            const T ty = fl(a.tauy, RT_TAUY, off, true);
             Fy = Fy + dv(a.has_tau_scale ? a.tau_scale * ty : ty, a.RHO0 * Dv);
         }
-        if (a.has_oy) Fy = Fy + (T(2) * (fl(a.oRy, off, ov) * d.oc[1] - fl(a.oIy, off, ov) * d.os[1])) * T(ov);
+        if (a.has_oy) Fy = Fy + (T(2) * (fl(a.oRy, -1, off, ov) * d.oc[1] - fl(a.oIy, -1, off, ov) * d.os[1])) * T(ov);
         if (a.nstream) apply_streams<T>(a, d, off, ou, ov, oe, Du, Dv, Fx, Fy, Fe);
 
         // SWM_timestep_nonlinear (src/swm_timestep_module.f90:349-443)
@@ -749,7 +839,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
         const int ipx = i + 1 == Nx ? 0 : i + 1, imx = i == 0 ? Nx - 1 : i - 1, jmx = j == 0 ? Ny - 1 : j - 1;
         if (oe) {
             const T t1 = dv(-(MUe - MUk), a.dL);
-            const T t2 = dv(MVn * rt(a.cv, jp) - MVk * rt(a.cv, j), a.dT);
+            const T t2 = dv(MVn * rv(RT_CV, 1) - MVk * rv(RT_CV, 0), a.dT);
             T tt = t1 - t2;
             if constexpr (VAR == VAR_MS) {
                 // src/swm_timestep_module.f90:359-365.  The first interpolation is
@@ -767,9 +857,9 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                 const T E3 = interp_noland2(lds.et[k], lds.et[ks], oe, OE(s_m[ks]));
                 const T vbn = fldg(a.vbs, (size_t)jp * P + i, jp, OV(s_m[kn]) != 0);
                 const T vbk = fldg(a.vbs, off, j, ov != 0);
-                tt = tt - dv((rt(a.cv, jp) * E2) * vbn - (rt(a.cv, j) * E3) * vbk, a.dT);
+                tt = tt - dv((rv(RT_CV, 1) * E2) * vbn - (rv(RT_CV, 0) * E3) * vbk, a.dT);
             }
-            Ge = dv(tt, rt(a.ace, j)) + Fe;
+            Ge = dv(tt, rv(RT_ACE, 0)) + Fe;
         }
         if (ou) {
             const T P2 = T(0.5) * s_Pot[kn] + T(0.5) * s_Pot[k];
@@ -783,19 +873,19 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                 const T vbk = fldg(a.vbs, off, j, ov != 0);
                 g = g + Z2 * (((T(0.25) * vbn + T(0.25) * vbw) + T(0.25) * vbnw) + T(0.25) * vbk);
             }
-            g = g - dv(s_Ed[k] - s_Ed[kw], rt(a.edu, j));
+            g = g - dv(s_Ed[k] - s_Ed[kw], rv(RT_EDU, 0));
             if (a.quad) {
                 const T vi = ((T(0.25) * s_v[kn] + T(0.25) * s_v[kw]) + T(0.25) * s_v[knw]) + T(0.25) * s_v[k];
                 const T uc = s_u[k];
-                T qf = fl(a.gsu, off, true) * sqrt(uc * uc + (vi * vi)) * uc;
-                if (a.barotropic) qf = dv(qf, fl(a.Hu, off, true));
+                T qf = fl(a.gsu, -1, off, true) * sqrt(uc * uc + (vi * vi)) * uc;
+                if (a.barotropic) qf = dv(qf, fl(a.Hu, -1, off, true));
                 g = g - qf;
             }
             if (a.mix) {
                 const int ot = (oh + OH(s_m[kn])) < 1 ? 1 : (oh + OH(s_m[kn]));
-                const T* c = a.lmu + j * 6;
-                const T c0 = ot == 1 ? rt(c, 0) : rt(c, 3), c1 = ot == 1 ? rt(c, 1) : rt(c, 4),
-                        c2 = ot == 1 ? rt(c, 2) : rt(c, 5);
+                const T c0 = ot == 1 ? rv(RT_LMU, 0) : rv(RT_LMU + 3, 0),
+                        c1 = ot == 1 ? rv(RT_LMU + 1, 0) : rv(RT_LMU + 4, 0),
+                        c2 = ot == 1 ? rv(RT_LMU + 2, 0) : rv(RT_LMU + 5, 0);
                 Lu = dv((c0 * (s_pll[k] - s_pll[kw]) + c1 * s_plt[kn]) + c2 * s_plt[k], Du);
                 g = g + Lu;
             }
@@ -817,15 +907,15 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             if (a.quad) {
                 const T ui = ((T(0.25) * s_u[ke] + T(0.25) * s_u[ks]) + T(0.25) * s_u[k]) + T(0.25) * s_u[kse];
                 const T vc = s_v[k];
-                T qf = fl(a.gsv, off, true) * sqrt(vc * vc + (ui * ui)) * vc;
-                if (a.barotropic) qf = dv(qf, fl(a.Hv, off, true));
+                T qf = fl(a.gsv, -1, off, true) * sqrt(vc * vc + (ui * ui)) * vc;
+                if (a.barotropic) qf = dv(qf, fl(a.Hv, -1, off, true));
                 g = g - qf;
             }
             if (a.mix) {
                 const int ot = (oe + OE(s_m[ks])) < 1 ? 1 : (oe + OE(s_m[ks]));
-                const T* c = a.lmv + j * 6;
-                const T c0 = ot == 1 ? rt(c, 0) : rt(c, 3), c1 = ot == 1 ? rt(c, 1) : rt(c, 4),
-                        c2 = ot == 1 ? rt(c, 2) : rt(c, 5);
+                const T c0 = ot == 1 ? rv(RT_LMV, 0) : rv(RT_LMV + 3, 0),
+                        c1 = ot == 1 ? rv(RT_LMV + 1, 0) : rv(RT_LMV + 4, 0),
+                        c2 = ot == 1 ? rv(RT_LMV + 2, 0) : rv(RT_LMV + 5, 0);
                 Lv = dv((c0 * (s_plt[ke] - s_plt[k]) + c1 * s_pll[k]) + c2 * s_pll[ks], Dv);
                 g = g + Lv;
             }
@@ -864,11 +954,11 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             vn = madd(vn, d.dtc3, Gv3);
             en = madd(en, d.dtc3, Ge3);
         }
-        un = dv(un, fl(a.imu, off, ou));
-        vn = dv(vn, fl(a.imv, off, ov));
-        en = dv(en, fl(a.ime, off, oe));
+        un = dv(un, fl(a.imu, RT_IMU, off, ou));
+        vn = dv(vn, fl(a.imv, RT_IMV, off, ov));
+        en = dv(en, fl(a.ime, RT_IME, off, oe));
         if (FN) {   // SWM_timestep_step: where (eta < minD - H) eta = minD - H
-            const T lim = a.minD - fl(a.He, off, oe);
+            const T lim = a.minD - fl(a.He, RT_HE, off, oe);
             if (en < lim) en = lim;
         }
         gst(st, (PL_U + 1 - d.cur) * pl + off, un);
@@ -894,22 +984,22 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                 const T kap = a.kappa[n];
                 const T hue = DUat(ke);   // interp2 of D(i+1,j), D(i,j)
                 const T hvn = DVat(kn);   // interp2 of D(i,j+1), D(i,j)
-                const T c9 = oue ? rt(a.t9, j) : T(0.0);
-                const T c10 = ou ? -rt(a.t9, j) : T(-0.0);
-                const T c11 = ovn ? rt(a.t11, j) : T(0.0);
-                const T c12 = ov ? -rt(a.t12, j) : T(-0.0);
+                const T c9 = oue ? rv(RT_T9, 0) : T(0.0);
+                const T c10 = ou ? -rv(RT_T9, 0) : T(-0.0);
+                const T c11 = ovn ? rv(RT_T11, 0) : T(0.0);
+                const T c12 = ov ? -rv(RT_T12, 0) : T(-0.0);
                 diff = ((((kap * hue) * (Ce - C)) * c9 + ((kap * Du) * (C - Cw)) * c10) +
                         ((kap * hvn) * (Cn - C)) * c11) + ((kap * Dv) * (C - Cs)) * c12;
             }
             const T forcing = C * Fe;
-            const T c1 = ta(true, oue & oee, oee + oe, a.tA);
-            const T c2 = ta(false, ou & oew, oew + oe, a.tA);
-            const T c3 = ta(true, ovn & oen, oen + oe, a.tB);
-            const T c4 = ta(false, ov & oes, oes + oe, a.tC);
-            const T c5 = ta(true, oue & oe, oee + oe, a.tA);
-            const T c6 = ta(false, ou & oe, oew + oe, a.tA);
-            const T c7 = ta(true, ovn & oe, oen + oe, a.tB);
-            const T c8 = ta(false, ov & oe, oes + oe, a.tC);
+            const T c1 = ta(true, oue & oee, oee + oe, RT_TA);
+            const T c2 = ta(false, ou & oew, oew + oe, RT_TA);
+            const T c3 = ta(true, ovn & oen, oen + oe, RT_TB);
+            const T c4 = ta(false, ov & oes, oes + oe, RT_TC);
+            const T c5 = ta(true, oue & oe, oee + oe, RT_TA);
+            const T c6 = ta(false, ou & oe, oew + oe, RT_TA);
+            const T c7 = ta(true, ovn & oe, oen + oe, RT_TB);
+            const T c8 = ta(false, ov & oe, oes + oe, RT_TC);
             T g = (CHe * ume) * c1 + (CHw * um) * c2;
             g = g + (CHn * vmn) * c3;
             g = g + (CHs * vm) * c4;
@@ -918,12 +1008,12 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             g = g + (CH * vmn) * c7;
             g = g + (CH * vm) * c8;
             g = g + diff;
-            g = g - (fl(a.gC[n], off, oe) * s_D[k]) * (C - fl(a.C0[n], off, oe));
+            g = g - (fl(a.gC[n], RT_TRC + RT_PER_TRC * n + 2, off, oe) * s_D[k]) * (C - fl(a.C0[n], RT_TRC + RT_PER_TRC * n + 4, off, oe));
             g = g + forcing;
             T chn = CH + d.dtc1 * pt[q][n][0];
             if (d.tstep >= 2) chn = madd(chn, d.dtc2, pt[q][n][1]);
             if (d.tstep >= 3) chn = madd(chn, d.dtc3, g);
-            chn = dv(chn, fl(a.timpl[n], off, oe));
+            chn = dv(chn, fl(a.timpl[n], RT_TRC + RT_PER_TRC * n, off, oe));
             gst(tb, (1 - d.cur) * pl + off, chn);
             gst(tb, (2 + r2) * pl + off, g);
         }
@@ -942,7 +1032,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
 template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, bool LATE = false, typename L>
 __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) {
     using S = TileShapeC<TX, TY>;
-    TileRegs<T, S::NE, S::NO, (NTRC > 0 ? NTRC : 1)> R;
+    TileRegs<T, S::NE, S::NO, (NTRC > 0 ? NTRC : 1), RowTabC<TX, TY, NTRC>::NQ> R;
     tile_load<T, VAR, NTRC, TX, TY, DIAG, !LATE>(d, t, R);
     tile_stage<T, VAR, NTRC, TX, TY, DIAG>(d, t, R, lds);
     if (LATE) tile_load_hist<T, VAR, NTRC, TX, TY, DIAG>(d, t, R);
@@ -954,8 +1044,9 @@ template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, bool LATE = 
 __global__ void __launch_bounds__(256)
 step_kernel(const Dyn<T> d) {
     using DD = LdsDuDv<T, TX, TY>;
+    using RTC = RowTabC<TX, TY, NTRC>;
     __shared__ TileLds<T, (NTRC > 0 ? NTRC : 1), (TX + 2) * (TY + 2), TY + 2,
-                       (VAR == VAR_MS ? (TX + 2) * (TY + 2) : 1), DD::DUE, DD::DVE> lds;
+                       (VAR == VAR_MS ? (TX + 2) * (TY + 2) : 1), DD::DUE, DD::DVE, RTC::LR, RTC::LK> lds;
     tile_body<T, VAR, NTRC, TX, TY, DIAG, LATE>(d, xcd_tile(blockIdx.x, d.nblk), lds);
 }
 
@@ -969,12 +1060,13 @@ step_kernel_p(const Dyn<T> d) {
     using S = TileShapeC<TX, TY>;
     constexpr int NTR = NTRC > 0 ? NTRC : 1;
     using DD = LdsDuDv<T, TX, TY>;
-    __shared__ TileLds<T, NTR, S::E, S::EH, 1, DD::DUE, DD::DVE> lds;
+    using RTC = RowTabC<TX, TY, NTRC>;
+    __shared__ TileLds<T, NTR, S::E, S::EH, 1, DD::DUE, DD::DVE, RTC::LR, RTC::LK> lds;
     const int b = xcd_tile(blockIdx.x, gridDim.x);
     const int t0 = b * d.tpb;
     const int t1 = t0 + d.tpb < d.nblk ? t0 + d.tpb : d.nblk;
     if (t0 >= t1) return;   // whole block
-    TileRegs<T, S::NE, S::NO, NTR> R;
+    TileRegs<T, S::NE, S::NO, NTR, RTC::NQ> R;
     tile_load<T, VAR, NTRC, TX, TY, false>(d, t0, R);
     for (int t = t0; t < t1; ++t) {
         if (t > t0) __syncthreads();   // phase 3 of the previous tile is done with LDS
@@ -1024,7 +1116,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 batch_kernel(const Member<T>* __restrict__ mem, const MemberStep* __restrict__ stp, const int* __restrict__ owner,
              int nwork) {
     using DD = LdsDuDv<T, TX, TY>;
-    __shared__ TileLds<T, (NTRC_MAX > 0 ? NTRC_MAX : 1), (TX + 2) * (TY + 2), TY + 2, 1, DD::DUE, DD::DVE> lds;
+    using RTC = RowTabC<TX, TY, NTRC_MAX>;
+    __shared__ TileLds<T, (NTRC_MAX > 0 ? NTRC_MAX : 1), (TX + 2) * (TY + 2), TY + 2, 1, DD::DUE, DD::DVE, RTC::LR,
+                       RTC::LK> lds;
     const int w = xcd_tile(blockIdx.x, nwork);
     const int lo = owner[w];   // member whose work range holds w
     const Member<T>& m = mem[lo];
