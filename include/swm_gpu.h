@@ -186,12 +186,12 @@ int swm_gpu_avg_read(swm_ctx *c, int32_t avg_id, double *host_out, double *weigh
 
 /* Tuning knobs and the autotuner.  No counterpart in the reference (its loops
  * have no launch shapes); they serve callers that pick kernel shapes for their
- * grids.  Shapes 0..swm_gpu_kernel_shape_count()-1: 0 = 64x4 LDS tiles
- * (default for FP64 grids of 3072^2 cells and more), 1 = 32x8 (default for
- * other grids above 2^21 cells), 2 = 64x8, 3 = 128x4, 4 = 32x16, 5..8 =
- * marching wavefronts (automatic, 8, 16, 4 rows), 9, 10 = persistent 64x4,
- * 32x8 tiles, 11, 12 = 32x8, 64x4 tiles loading the AB history after the
- * halo (11 is the default up to 2^21 cells); every shape gives the same FP64
+ * grids.  Shapes 0..swm_gpu_kernel_shape_count()-1: 0 = 64x4 LDS tiles,
+ * 1 = 32x8 (default for FP32 grids above 2^22 cells), 2 = 64x8, 3 = 128x4,
+ * 4 = 32x16, 5..8 = marching wavefronts (automatic, 8, 16, 4 rows), 9, 10 =
+ * persistent 64x4, 32x8 tiles, 11, 12 = 32x8, 64x4 tiles loading the AB
+ * history after the halo (11 is the default for every other grid); every
+ * shape gives the same FP64
  * results bit for bit.  swm_gpu_tune times
  * `steps` steps of every shape from a device snapshot of the state, restores
  * the state exactly, selects the fastest and returns it (times_ms, if not

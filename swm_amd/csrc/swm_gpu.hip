@@ -472,6 +472,7 @@ int write_statics_t(swm_ctx* c) {
     a.has_ox = d.osc_re[0] ? 1 : 0;
     a.G = (T)d.G; a.A = (T)d.A; a.dL = (T)d.dLambda; a.dT = (T)d.dTheta; a.RHO0 = (T)d.RHO0;
     a.minD = (T)d.minD; a.lbc = (T)d.lbc; a.tau_scale = (T)d.tau_scale; a.AdT = (T)(d.A * d.dTheta);
+    a.rdL = (T)(1.0 / d.dLambda); a.rdT = (T)(1.0 / d.dTheta); a.rAdT = (T)(1.0 / (d.A * d.dTheta));
     a.eps = std::is_same<T, double>::value ? (T)2.220446049250313e-16 : (T)1.1920929e-07f;
     a.mask = c->mask;
     a.He = F<T>(c->fields[F_HE]); a.Hu = F<T>(c->fields[F_HU]); a.Hv = F<T>(c->fields[F_HV]);
@@ -883,13 +884,12 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
     c->Ny = d->Ny;
     c->fp32 = d->precision == SWM_FP32;
     c->esz = c->fp32 ? 4 : 8;
-    // Default shape from measured step times on MI355X: 32x8 tiles carry less
-    // halo per output cell and win in FP32 at every size and in FP64 up to
-    // about 3072^2 cells; past that the FP64 kernel is HBM-bound and the
-    // longer 512-byte rows of 64x4 tiles win.  Grids of up to 2^21 cells are
-    // only a few rounds of resident blocks: there the late-history order wins.
+    // Default shape from measured step times on MI355X (1024^2 .. 4096^2):
+    // 32x8 tiles carry less halo per output cell than 64x4 and win at every
+    // size; loading the AB history after the halo stage wins in FP64 at every
+    // size and in FP32 up to 2^22 cells.
     const long long cells = (long long)d->Nx * d->Ny;
-    c->tile = (c->fp32 || cells < 3072LL * 3072LL) ? (cells <= (1LL << 21) ? 11 : 1) : 0;
+    c->tile = (c->fp32 && cells > (1LL << 22)) ? 1 : 11;
     if (const char* e = getenv("SWM_GPU_TILE")) c->tile = std::max(0, std::min(kNumTiles - 1, atoi(e)));
     if (d->variant == SWM_LINEARISED_MEAN_STATE && kTiles[c->tile].march) c->tile = 1;   // LDS tile kernels only
     if (const char* e = getenv("SWM_GPU_GRAPH")) c->use_graph = atoi(e) != 0;

@@ -40,14 +40,15 @@ __device__ __forceinline__ T fld(const Fld<T>& f, size_t off, int j, bool ocean)
 // Packed per-row values that phase 3 of the tile kernels reads, RT_N per row
 // (Statics::rowtab): built on the device from the per-row tables and the
 // row-compressed fields whenever the statics are written.  A field stored as
-// a full array leaves its two entries 0 and is read from the array.  Tracer n
-// has RT_PER_TRC entries from RT_TRC + n * RT_PER_TRC: timpl, gC, C0 (ocean,
-// land each).
+// a full array leaves its two entries 0 and is read from the array.  RT_R*
+// are the correctly rounded reciprocals of the divisors (dvr).  Tracer n has
+// RT_PER_TRC entries from RT_TRC + n * RT_PER_TRC: timpl, gC, C0, 1 / timpl
+// (ocean, land each).
 enum : int {
     RT_CV = 0, RT_ACE = 1, RT_EDU = 2, RT_LMU = 3, RT_LMV = 9, RT_T9 = 15, RT_T11 = 16, RT_T12 = 17,
     RT_TA = 18, RT_TB = 20, RT_TC = 22, RT_FEC = 24, RT_FXC = 26, RT_FYC = 28, RT_IMU = 30, RT_IMV = 32,
-    RT_IME = 34, RT_HE = 36, RT_TAUX = 38, RT_TAUY = 40, RT_TRC = 42, RT_PER_TRC = 6,
-    RT_N = RT_TRC + 2 * RT_PER_TRC
+    RT_IME = 34, RT_HE = 36, RT_TAUX = 38, RT_TAUY = 40, RT_RACE = 42, RT_REDU = 43, RT_RIMU = 44,
+    RT_RIMV = 46, RT_RIME = 48, RT_TRC = 50, RT_PER_TRC = 8, RT_N = RT_TRC + 2 * RT_PER_TRC
 };
 
 // Static inputs of the step kernel, resident in device memory and read through
@@ -59,6 +60,7 @@ struct Statics {
     int mix, quad, barotropic, has_tau_scale;
     int has_taux, has_tauy, has_ox, has_oy, has_oe;
     T G, A, dL, dT, RHO0, minD, lbc, tau_scale, AdT, eps;
+    T rdL, rdT, rAdT;    // 1/dL, 1/dT, 1/AdT (correctly rounded, for dvr)
     const uint8_t* mask;
     Fld<T> He, Hu, Hv, imu, imv, ime, gsu, gsv, Fxc, Fyc, Fec, taux, tauy, oRx, oIx, oRy, oIy, oRe, oIe;
     // per-row tables
@@ -97,6 +99,7 @@ __global__ void __launch_bounds__(256) pack_rowtab(const Statics<T>* __restrict_
     const Statics<T>& a = *sp;
     const int j = x / RT_N, k = x % RT_N;
     auto rows = [&](const Fld<T>& f, int s) -> T { return f.rows ? f.rows[2 * j + s] : T(0); };
+    auto rcp = [](T b) -> T { return b != T(0) ? T(1) / b : T(0); };   // correctly rounded
     T v;
     if (k == RT_CV) v = a.cv[j];
     else if (k == RT_ACE) v = a.ace[j];
@@ -117,10 +120,16 @@ __global__ void __launch_bounds__(256) pack_rowtab(const Statics<T>* __restrict_
     else if (k < RT_HE) v = rows(a.ime, k - RT_IME);
     else if (k < RT_TAUX) v = rows(a.He, k - RT_HE);
     else if (k < RT_TAUY) v = rows(a.taux, k - RT_TAUX);
-    else if (k < RT_TRC) v = rows(a.tauy, k - RT_TAUY);
+    else if (k < RT_RACE) v = rows(a.tauy, k - RT_TAUY);
+    else if (k == RT_RACE) v = rcp(a.ace[j]);
+    else if (k == RT_REDU) v = rcp(a.edu[j]);
+    else if (k < RT_RIMV) v = rcp(rows(a.imu, k - RT_RIMU));
+    else if (k < RT_RIME) v = rcp(rows(a.imv, k - RT_RIMV));
+    else if (k < RT_TRC) v = rcp(rows(a.ime, k - RT_RIME));
     else {
         const int n = (k - RT_TRC) / RT_PER_TRC, r = (k - RT_TRC) % RT_PER_TRC;
-        v = rows(r < 2 ? a.timpl[n] : (r < 4 ? a.gC[n] : a.C0[n]), r & 1);
+        v = rows(r < 2 || r >= 6 ? a.timpl[n] : (r < 4 ? a.gC[n] : a.C0[n]), r & 1);
+        if (r >= 6) v = rcp(v);
     }
     out[x] = v;
 }
@@ -197,6 +206,22 @@ __device__ __forceinline__ T interp_noland2(T D1, T D2, int m1, int m2) {
 // divides through the hardware reciprocal and fuses multiply-adds.
 __device__ __forceinline__ double dv(double a, double b) { return a / b; }
 __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+// a / b where y = 1 / b correctly rounded is at hand (constant and per-row
+// divisors).  FP64 gives exactly a / b: q0 = a*y, a remainder step
+// r = a - b*q (exact with an FMA) and q + r*y makes q faithful, and a second
+// step rounds it correctly (Markstein's theorem, P. W. Markstein, IBM J. Res.
+// Dev. 34(1):111-119, 1990); zero and infinite quotients are q0, which
+// carries their sign.  Five FMA-pipe operations and a class test in place of
+// the scaled reciprocal iteration of a full division.
+__device__ __forceinline__ double dvr(double a, double b, double y) {
+    const double q0 = a * y;
+    double r = __builtin_fma(-b, q0, a);
+    const double q1 = __builtin_fma(r, y, q0);
+    r = __builtin_fma(-b, q1, a);
+    const double q2 = __builtin_fma(r, y, q1);
+    return __builtin_amdgcn_class(q0, 0x264) ? q0 : q2;   // 0x264: -inf, -0, +0, +inf
+}
+__device__ __forceinline__ float dvr(float a, float b, float y) { return a * y; }
 __device__ __forceinline__ double madd(double acc, double x, double y) { return acc + x * y; }
 __device__ __forceinline__ float madd(float acc, float x, float y) { return __builtin_fmaf(x, y, acc); }
 
@@ -257,7 +282,7 @@ struct TileLds {
     T Du[DUE], Dv[DVE];  // interp2(D, eta2u/eta2v_noland) of every halo cell (when DUE / DVE = E)
     T CH[NTR][E], C[NTR][E];
     // per-row tables of the EH tile rows (cu also of the row above them)
-    T za[EH][2], cu[EH + 1], cH[EH], fH[EH], pltc[EH][12], pllc[EH][12];
+    T za[EH][2], cu[EH + 1], cH[EH], rcH[EH], fH[EH], pltc[EH][12], pllc[EH][12];
     T rt[RTR][RTK_];     // rowtab rows of the output rows and the row after them (RowTabC)
     uint8_t m[E];
 };
@@ -593,6 +618,7 @@ __device__ __forceinline__ void tile_stage(const Dyn<T>& d, const int t, const R
         lds.za[tid][1] = tz1;
         lds.cu[tid + 1] = tcu;
         lds.cH[tid] = tcH;
+        lds.rcH[tid] = T(1) / tcH;   // correctly rounded (dvr)
         lds.fH[tid] = tfH;
     }
     if (tid == NT - 1) lds.cu[0] = tcu0;
@@ -687,9 +713,9 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                     za_ = lds.za[ly][OH(m) ? 0 : 1] * (s_v[e] - s_v[kw]);
                 if (OU(m) + OU(s_m[ks]) != 0) {
                     const T bcH = OH(m) ? T(1) : a.lbc;
-                    zb = dv(bcH * (lds.cu[ly + 1] * s_u[e] - lds.cu[ly] * s_u[ks]), a.AdT);
+                    zb = dvr(bcH * (lds.cu[ly + 1] * s_u[e] - lds.cu[ly] * s_u[ks]), a.AdT, a.rAdT);
                 }
-                zeta = za_ - dv(zb, lds.cH[ly]);
+                zeta = za_ - dvr(zb, lds.cH[ly], lds.rcH[ly]);
             }
             // computePotVort (src/swm_timestep_module.f90:230-257)
             const T pD = Dh > a.eps ? Dh : a.eps;
@@ -787,6 +813,13 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             const T fo = rv(e, 0), fl_ = rv(e + 1, 0);
             return ocean ? fo : fl_;
         };
+        // x / (static field at (i, j)); e, er: its rowtab entry and that of its reciprocal
+        auto flq = [&](T x, auto& f, int e, int er, size_t at, bool ocean) -> T {
+            if (f.full) return dv(x, gld(f.full, at));
+            const T b = ocean ? rv(e, 0) : rv(e + 1, 0);
+            const T y = ocean ? rv(er, 0) : rv(er + 1, 0);
+            return dvr(x, b, y);
+        };
         // TRC_coeff magnitude (trc_adv) from the rowtab pair at e
         auto ta = [&](bool neg, int nummask, int den, int e) -> T {
             const T v1 = rv(e, 0), v2 = rv(e + 1, 0);
@@ -838,8 +871,8 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
         // neighbours' offsets for the basic-state terms (LINEARISED_MEAN_STATE)
         const int ipx = i + 1 == Nx ? 0 : i + 1, imx = i == 0 ? Nx - 1 : i - 1, jmx = j == 0 ? Ny - 1 : j - 1;
         if (oe) {
-            const T t1 = dv(-(MUe - MUk), a.dL);
-            const T t2 = dv(MVn * rv(RT_CV, 1) - MVk * rv(RT_CV, 0), a.dT);
+            const T t1 = dvr(-(MUe - MUk), a.dL, a.rdL);
+            const T t2 = dvr(MVn * rv(RT_CV, 1) - MVk * rv(RT_CV, 0), a.dT, a.rdT);
             T tt = t1 - t2;
             if constexpr (VAR == VAR_MS) {
                 // src/swm_timestep_module.f90:359-365.  The first interpolation is
@@ -852,14 +885,14 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                 const T E0 = interp_noland2(lds.et[k], lds.et[kw], oe, OE(s_m[kw]));
                 const T ube = fldg(a.ubs, (size_t)j * P + ipx, j, OU(s_m[ke]) != 0);
                 const T ubk = fldg(a.ubs, off, j, ou != 0);
-                tt = tt - dv(E1 * ube - E0 * ubk, a.dL);
+                tt = tt - dvr(E1 * ube - E0 * ubk, a.dL, a.rdL);
                 const T E2 = interp_noland2(lds.et[kn], lds.et[k], OE(s_m[kn]), oe);
                 const T E3 = interp_noland2(lds.et[k], lds.et[ks], oe, OE(s_m[ks]));
                 const T vbn = fldg(a.vbs, (size_t)jp * P + i, jp, OV(s_m[kn]) != 0);
                 const T vbk = fldg(a.vbs, off, j, ov != 0);
-                tt = tt - dv((rv(RT_CV, 1) * E2) * vbn - (rv(RT_CV, 0) * E3) * vbk, a.dT);
+                tt = tt - dvr((rv(RT_CV, 1) * E2) * vbn - (rv(RT_CV, 0) * E3) * vbk, a.dT, a.rdT);
             }
-            Ge = dv(tt, rv(RT_ACE, 0)) + Fe;
+            Ge = dvr(tt, rv(RT_ACE, 0), rv(RT_RACE, 0)) + Fe;
         }
         if (ou) {
             const T P2 = T(0.5) * s_Pot[kn] + T(0.5) * s_Pot[k];
@@ -873,7 +906,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                 const T vbk = fldg(a.vbs, off, j, ov != 0);
                 g = g + Z2 * (((T(0.25) * vbn + T(0.25) * vbw) + T(0.25) * vbnw) + T(0.25) * vbk);
             }
-            g = g - dv(s_Ed[k] - s_Ed[kw], rv(RT_EDU, 0));
+            g = g - dvr(s_Ed[k] - s_Ed[kw], rv(RT_EDU, 0), rv(RT_REDU, 0));
             if (a.quad) {
                 const T vi = ((T(0.25) * s_v[kn] + T(0.25) * s_v[kw]) + T(0.25) * s_v[knw]) + T(0.25) * s_v[k];
                 const T uc = s_u[k];
@@ -903,7 +936,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                 const T ubse = fldg(a.ubs, (size_t)jmx * P + ipx, jmx, OU(s_m[kse]) != 0);
                 g = g - Z2 * (((T(0.25) * ube + T(0.25) * ubs_) + T(0.25) * ubk) + T(0.25) * ubse);
             }
-            g = g - dv(s_Ed[k] - s_Ed[ks], a.AdT);
+            g = g - dvr(s_Ed[k] - s_Ed[ks], a.AdT, a.rAdT);
             if (a.quad) {
                 const T ui = ((T(0.25) * s_u[ke] + T(0.25) * s_u[ks]) + T(0.25) * s_u[k]) + T(0.25) * s_u[kse];
                 const T vc = s_v[k];
@@ -954,9 +987,9 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             vn = madd(vn, d.dtc3, Gv3);
             en = madd(en, d.dtc3, Ge3);
         }
-        un = dv(un, fl(a.imu, RT_IMU, off, ou));
-        vn = dv(vn, fl(a.imv, RT_IMV, off, ov));
-        en = dv(en, fl(a.ime, RT_IME, off, oe));
+        un = flq(un, a.imu, RT_IMU, RT_RIMU, off, ou);
+        vn = flq(vn, a.imv, RT_IMV, RT_RIMV, off, ov);
+        en = flq(en, a.ime, RT_IME, RT_RIME, off, oe);
         if (FN) {   // SWM_timestep_step: where (eta < minD - H) eta = minD - H
             const T lim = a.minD - fl(a.He, RT_HE, off, oe);
             if (en < lim) en = lim;
@@ -1013,7 +1046,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             T chn = CH + d.dtc1 * pt[q][n][0];
             if (d.tstep >= 2) chn = madd(chn, d.dtc2, pt[q][n][1]);
             if (d.tstep >= 3) chn = madd(chn, d.dtc3, g);
-            chn = dv(chn, fl(a.timpl[n], RT_TRC + RT_PER_TRC * n, off, oe));
+            chn = flq(chn, a.timpl[n], RT_TRC + RT_PER_TRC * n, RT_TRC + RT_PER_TRC * n + 6, off, oe);
             gst(tb, (1 - d.cur) * pl + off, chn);
             gst(tb, (2 + r2) * pl + off, g);
         }

@@ -2,7 +2,7 @@
 10 000 steps (at dt = 18 s, the largest stable whole-second step; the stated
 30 s overflows) and keeps the flux-form volume sum(eta * cos) to rounding in
 FP64 and within the FP32 bar; the 2048 x 1024 elliptic-workload basin and an
-8192 x 4096 basin (64 x 4 tiles, 33.5 M cells) are bit-identical to the
+8192 x 4096 basin (33.5 M cells) are bit-identical to the
 oracle for a few steps."""
 import numpy as np
 import pytest
@@ -52,7 +52,7 @@ def test_elliptic_workload_grid_bit_identical(gpu_lib):
 
 
 def test_large_grid_bit_identical(gpu_lib):
-    """8192 x 4096 FP64 (64 x 4 tiles by default at this size), 3 steps."""
+    """8192 x 4096 FP64 (the default 32 x 8 late-history tiles), 3 steps."""
     big = configs.basin(8192, 4096, 5e3, 5e3, variant=host.FULLY_NONLINEAR, H0=500.0, dt=10.0, Ah=100.0, f0=1e-4,
                         beta=2e-11, drag=1e-6, lateral_mixing=True, land_fraction=0.03, seed=5)
     host.add_windstress(big, host.COMP_X, configs.double_gyre_tau(big, 0.1))

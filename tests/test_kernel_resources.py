@@ -11,7 +11,7 @@ from conftest import ROOT
 
 REPORT = os.path.join(ROOT, "swm_amd", "libswmgpu.resources.txt")
 HOT = [
-    "_ZN3swm11step_kernelIdLi1ELi1ELi32ELi8ELb0ELb1EEEvNS_3DynIT_EE",   # FP64 default up to 2^21 cells
+    "_ZN3swm11step_kernelIdLi1ELi1ELi32ELi8ELb0ELb1EEEvNS_3DynIT_EE",   # FP64 default
     "_ZN3swm11step_kernelIdLi1ELi1ELi32ELi8ELb0ELb0EEEvNS_3DynIT_EE",
     "_ZN3swm11step_kernelIdLi1ELi1ELi64ELi4ELb0ELb0EEEvNS_3DynIT_EE",
     "_ZN3swm11step_kernelIfLi1ELi1ELi32ELi8ELb0ELb1EEEvNS_3DynIT_EE",
