@@ -1657,7 +1657,8 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
     constexpr int H = 2 * K, W = TX + 2 * H, HH = TY + 2 * H, E = W * HH;
     __shared__ T s_chi[E];
     __shared__ T s_B[E];
-    __shared__ uint8_t s_m[E];
+    __shared__ uint8_t s_m[E];   // ocean masks; bit 7 = colour ((i + j) % 2) of the cell
+    __shared__ int s_row[HH];    // global row of each staged row
     __shared__ double red[256 / 64];
     const int Nx = a.Nx, Ny = a.Ny, P = a.P;
     const int i0 = (blockIdx.x % ntx) * TX, j0 = (blockIdx.x / ntx) * TY;
@@ -1671,25 +1672,26 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
     }
     for (int e = tid; e < E; e += 256) {
         const int lx = e % W, ly = e / W;
-        const size_t off = (size_t)wrap(j0 - H + ly, Ny) * P + wrap(i0 - H + lx, Nx);
+        const int gi = wrap(i0 - H + lx, Nx), gj = wrap(j0 - H + ly, Ny);
+        const size_t off = (size_t)gj * P + gi;
         s_chi[e] = chi_in[off];
         s_B[e] = a.B[off];
-        s_m[e] = a.mask[off];
+        s_m[e] = (uint8_t)(a.mask[off] | (((gi + gj) & 1) << 7));
+        if (lx == 0) s_row[ly] = gj;
     }
     __syncthreads();
     double acc = 0.0;
 #pragma unroll 1
     for (int hs = 0; hs < 2 * K; ++hs) {
         const int color = hs & 1;
-        const int lo = hs + 1, rw = W - 2 * lo, rh = HH - 2 * lo;
+        const int lo = hs + 1;
         const bool tail = a.accumulate && hs >= 2 * K - 2;
-        for (int q = tid; q < rw * rh; q += 256) {
-            const int lx = lo + q % rw, ly = lo + q / rw;
-            const int gi = wrap(i0 - H + lx, Nx), j = wrap(j0 - H + ly, Ny);
-            if (((gi + j) & 1) != color) continue;
-            const int e = ly * W + lx;
+        for (int e = tid; e < E; e += 256) {
+            const int lx = e % W, ly = e / W;   // W is a compile-time constant
+            if (lx < lo || lx > W - 1 - lo || ly < lo || ly > HH - 1 - lo) continue;
             const uint8_t m = s_m[e];
-            if (!(m & OC_E)) continue;
+            if ((m >> 7) != color || !(m & OC_E)) continue;
+            const int j = s_row[ly];
             const uint8_t mip = s_m[e + 1], mjp = s_m[e + W];
             const int ouip = (mip >> 1) & 1, ou = (m >> 1) & 1, ovjp = (mjp >> 2) & 1, ov = (m >> 2) & 1;
             const T c1 = ouip ? a.c1t[j] : T(0);
