@@ -229,7 +229,7 @@ struct swm_ctx {
     void* scratch = nullptr;     // one plane (divergence output)
     double* psi_buf[4] = {};     // FP64 planes: pu, pv, column sums, psi
     // SOR
-    void* sor_tab[4] = {};
+    void* sor_tab[5] = {};      // c1, c3, c4, c5 and 1/c5
     void* sor_relax = nullptr;
     int sor_nrelax = 0;
     double* sor_partial = nullptr;
@@ -238,7 +238,7 @@ struct swm_ctx {
     double* sor_norm = nullptr;
     void* sor_buf[2][2] = {};    // [fp32][chi|B]
     void* sor_alt[2] = {};       // second chi buffer of the blocked sweeps (ping-pong)
-    void* sor_ftab[2][4] = {};   // coefficient tables in the sweep precision
+    void* sor_ftab[2][5] = {};   // coefficient tables in the sweep precision
     void* dza = nullptr;
     swm_batch* batch = nullptr;  // batch this context is attached to
 };
@@ -450,7 +450,10 @@ int build_tables(swm_ctx* c) {
     c->sor_tab[1] = upload_vec(c, c3);
     c->sor_tab[2] = upload_vec(c, c4);
     c->sor_tab[3] = upload_vec(c, c5);
-    if (!c->dza || !c->sor_tab[0] || !c->sor_tab[1] || !c->sor_tab[2] || !c->sor_tab[3])
+    std::vector<double> c5r(c5.size());   // correctly rounded reciprocals for dvr (1/-0 = -inf)
+    for (size_t k = 0; k < c5.size(); ++k) c5r[k] = 1.0 / c5[k];
+    c->sor_tab[4] = upload_vec(c, c5r);
+    if (!c->dza || !c->sor_tab[0] || !c->sor_tab[1] || !c->sor_tab[2] || !c->sor_tab[3] || !c->sor_tab[4])
         return fail(c, "table upload failed", 3);
     return 0;
 }
@@ -1217,8 +1220,8 @@ int sor_prepare(swm_ctx* c, bool f32, int nblk) {
     }
     if (!c->sor_ftab[w][0]) {
         const int Ny = c->Ny;
-        const size_t sz[4] = {(size_t)Ny, (size_t)Ny, (size_t)Ny, 12 * (size_t)Ny};
-        for (int q = 0; q < 4; ++q) {
+        const size_t sz[5] = {(size_t)Ny, (size_t)Ny, (size_t)Ny, 12 * (size_t)Ny, 12 * (size_t)Ny};
+        for (int q = 0; q < 5; ++q) {
             if (f32 == c->fp32) { c->sor_ftab[w][q] = c->sor_tab[q]; continue; }
             // tables were uploaded in the context precision; convert on the host
             std::vector<double> h(sz[q]);
@@ -1260,7 +1263,7 @@ void sor_block_launch(swm_ctx* c, const void* chi_in, void* chi_out, void* dB, v
     a.Nx = c->Nx; a.Ny = c->Ny; a.P = c->P; a.color = 0; a.sweep = (int)l;
     a.chi = nullptr; a.B = (const T*)dB; a.mask = c->mask;
     a.relax = nullptr; a.nrelax = 0; a.omega = omega;
-    a.c1t = (const T*)tabs[0]; a.c3t = (const T*)tabs[1]; a.c4t = (const T*)tabs[2]; a.c5t = (const T*)tabs[3];
+    a.c1t = (const T*)tabs[0]; a.c3t = (const T*)tabs[1]; a.c4t = (const T*)tabs[2]; a.c5t = (const T*)tabs[3]; a.c5r = (const T*)tabs[4];
     a.partial = c->sor_partial;
     a.done = c->sor_done;
     a.residual_mode = 1;
@@ -1314,7 +1317,7 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
                 a.chi = (T*)dchi; a.B = (const T*)dB; a.mask = c->mask;
                 a.relax = ref_mode ? (const double*)c->sor_relax : nullptr;
                 a.nrelax = c->sor_nrelax; a.omega = omega;
-                a.c1t = (const T*)tabs[0]; a.c3t = (const T*)tabs[1]; a.c4t = (const T*)tabs[2]; a.c5t = (const T*)tabs[3];
+                a.c1t = (const T*)tabs[0]; a.c3t = (const T*)tabs[1]; a.c4t = (const T*)tabs[2]; a.c5t = (const T*)tabs[3]; a.c5r = (const T*)tabs[4];
                 a.partial = c->sor_partial + color * nblk;
                 a.done = c->sor_done;
                 a.residual_mode = ref_mode ? 0 : 1;

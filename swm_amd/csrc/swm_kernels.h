@@ -1582,6 +1582,7 @@ struct SorArgs {
     double omega;           // fixed-omega mode when relax == nullptr
     const T *c1t, *c3t, *c4t;   // [Ny] 1/x, cv(j+1)/y, cv(j)/y
     const T* c5t;               // [Ny][3][2][2] c5 by (ou(i+1)+ou(i), ov(j+1), ov(j))
+    const T* c5r;               // 1 / c5t, correctly rounded (dvr)
     double* partial;            // per-block partial sums of res^2 (or r^2)
     const int* done;            // set once converged: later sweeps return at once
     int residual_mode;          // 0: sum res^2 (reference), 1: sum (L chi - B)^2
@@ -1614,7 +1615,8 @@ __global__ void __launch_bounds__(256) sor_sweep(const SorArgs<T> a) {
                 const T c2 = ou ? a.c1t[j] : T(0);
                 const T c3 = ovjp ? a.c3t[j] : T(0);
                 const T c4 = ov ? a.c4t[j] : T(0);
-                const T c5 = a.c5t[((j * 3 + ouip + ou) * 2 + ovjp) * 2 + ov];
+                const int c5i = ((j * 3 + ouip + ou) * 2 + ovjp) * 2 + ov;
+                const T c5 = a.c5t[c5i];
                 T* chi = a.chi;
                 const T x = chi[off];
                 const T lhs = (((c1 * chi[(size_t)j * a.P + ip] + c2 * chi[(size_t)j * a.P + im]) +
@@ -1624,9 +1626,9 @@ __global__ void __launch_bounds__(256) sor_sweep(const SorArgs<T> a) {
                 if (a.relax) {
                     int h = 2 * (a.sweep - 1) + a.color;
                     if (h >= a.nrelax) h = a.nrelax - 1;
-                    res = T(a.relax[h]) * r / c5;
+                    res = dvr(T(a.relax[h]) * r, c5, a.c5r[c5i]);
                 } else {
-                    res = T(a.omega) * r / c5;
+                    res = dvr(T(a.omega) * r, c5, a.c5r[c5i]);
                 }
                 chi[off] = x - res;
                 const double q = a.residual_mode ? (double)r : (double)res;
@@ -1680,15 +1682,32 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
         if (lx == 0) s_row[ly] = gj;
     }
     __syncthreads();
+    // Without an odd periodic wrap inside the block the colours alternate
+    // along every staged row (colour = (p0 + lx + ly) % 2): each half-sweep
+    // then visits only the cells of its colour, two per thread fewer
+    // iterations; otherwise it walks the block and tests the colour bit.
+    static_assert(W % 2 == 0, "pairs of cells along a row");
+    const bool regular = ((Nx & 1) == 0 || (i0 - H >= 0 && i0 - H + W <= Nx)) &&
+                         ((Ny & 1) == 0 || (j0 - H >= 0 && j0 - H + HH <= Ny));
+    const int p0 = (i0 + j0) & 1;
     double acc = 0.0;
 #pragma unroll 1
     for (int hs = 0; hs < 2 * K; ++hs) {
         const int color = hs & 1;
         const int lo = hs + 1;
         const bool tail = a.accumulate && hs >= 2 * K - 2;
-        for (int e = tid; e < E; e += 256) {
-            const int lx = e % W, ly = e / W;   // W is a compile-time constant
+        const int n = regular ? HH * (W / 2) : E;
+        for (int q = tid; q < n; q += 256) {
+            int lx, ly;
+            if (regular) {
+                ly = q / (W / 2);
+                lx = 2 * (q % (W / 2)) + ((color + p0 + ly) & 1);
+            } else {
+                lx = q % W;
+                ly = q / W;
+            }
             if (lx < lo || lx > W - 1 - lo || ly < lo || ly > HH - 1 - lo) continue;
+            const int e = ly * W + lx;
             const uint8_t m = s_m[e];
             if ((m >> 7) != color || !(m & OC_E)) continue;
             const int j = s_row[ly];
@@ -1698,11 +1717,12 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
             const T c2 = ou ? a.c1t[j] : T(0);
             const T c3 = ovjp ? a.c3t[j] : T(0);
             const T c4 = ov ? a.c4t[j] : T(0);
-            const T c5 = a.c5t[((j * 3 + ouip + ou) * 2 + ovjp) * 2 + ov];
+            const int c5i = ((j * 3 + ouip + ou) * 2 + ovjp) * 2 + ov;
+            const T c5 = a.c5t[c5i];
             const T x = s_chi[e];
             const T lhs = (((c1 * s_chi[e + 1] + c2 * s_chi[e - 1]) + c3 * s_chi[e + W]) + c4 * s_chi[e - W]) + c5 * x;
             const T r = lhs - s_B[e];
-            const T res = T(a.omega) * r / c5;
+            const T res = dvr(T(a.omega) * r, c5, a.c5r[c5i]);
             s_chi[e] = x - res;
             if (tail && lx >= H && lx < H + TX && ly >= H && ly < H + TY && i0 + lx - H < Nx && j0 + ly - H < Ny)
                 acc += (double)r * (double)r;
