@@ -2,7 +2,9 @@
 used when the convergence test runs every 2 or 4 sweeps or a multiple): the
 same chi, bit for bit, as the oracle's sequential sweeps in FP64 and as the
 one-colour-per-launch sweeps in FP32, for sweep counts that leave a remainder,
-grids that no tile divides and a grid narrower than the tile's halo."""
+grids that no tile divides, a grid narrower than the tile's halo, and odd
+periodic extents (301 columns, 75 rows), where blocks clear of the wrap visit
+only their colour's cells and blocks across it test each cell's colour."""
 import numpy as np
 import pytest
 
@@ -26,7 +28,8 @@ def rhs(o, s, seed):
 
 
 @pytest.mark.parametrize("nx,ny,every,sweeps", [(96, 64, 4, 40), (130, 70, 4, 42), (130, 70, 2, 41),
-                                                (7, 5, 4, 12), (200, 33, 16, 50)])
+                                                (7, 5, 4, 12), (200, 33, 16, 50), (301, 40, 2, 31),
+                                                (96, 75, 4, 40)])
 def test_blocked_sweeps_equal_sequential_sweeps(gpu_lib, nx, ny, every, sweeps):
     s = basin(nx, ny, 0.08 if nx > 20 else 0.0, seed=nx + ny)
     o = oracle.Oracle(s)
