@@ -806,26 +806,32 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             if constexpr (UROW) return ldc(a.rowtab, (size_t)(dr ? jp : j) * RT_N + e);
             else return lds.rt[ly - 1 + dr][e];
         };
+        // rowtab entry (c ? e0 : e1) of row j: one LDS read at the selected
+        // index, or both scalar loads and a select
+        auto rvs = [&](bool c, int e0, int e1) -> T {
+            if constexpr (UROW) {
+                const T v0 = rv(e0, 0), v1 = rv(e1, 0);
+                return c ? v0 : v1;
+            } else {
+                return lds.rt[ly - 1][c ? e0 : e1];
+            }
+        };
         // static field at (i, j); e = its rowtab entry, -1 for fields without one
         auto fl = [&](auto& f, int e, size_t at, bool ocean) -> T {
             if (f.full) return gld(f.full, at);
             if (e < 0) return gld(f.rows, 2 * (size_t)j + (ocean ? 0 : 1));
-            const T fo = rv(e, 0), fl_ = rv(e + 1, 0);
-            return ocean ? fo : fl_;
+            return rvs(ocean, e, e + 1);
         };
         // x / (static field at (i, j)); e, er: its rowtab entry and that of its reciprocal
         auto flq = [&](T x, auto& f, int e, int er, size_t at, bool ocean) -> T {
             if (f.full) return dv(x, gld(f.full, at));
-            const T b = ocean ? rv(e, 0) : rv(e + 1, 0);
-            const T y = ocean ? rv(er, 0) : rv(er + 1, 0);
-            return dvr(x, b, y);
+            return dvr(x, rvs(ocean, e, e + 1), rvs(ocean, er, er + 1));
         };
         // TRC_coeff magnitude (trc_adv) from the rowtab pair at e
         auto ta = [&](bool neg, int nummask, int den, int e) -> T {
-            const T v1 = rv(e, 0), v2 = rv(e + 1, 0);
+            const T v = rvs(den == 1, e, e + 1);
             if (den == 0) return __builtin_nan("");
             if (!nummask) return neg ? T(-0.0) : T(0.0);
-            const T v = den == 1 ? v1 : v2;
             return neg ? -v : v;
         };
         const uint8_t m = s_m[k];
@@ -916,9 +922,8 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             }
             if (a.mix) {
                 const int ot = (oh + OH(s_m[kn])) < 1 ? 1 : (oh + OH(s_m[kn]));
-                const T c0 = ot == 1 ? rv(RT_LMU, 0) : rv(RT_LMU + 3, 0),
-                        c1 = ot == 1 ? rv(RT_LMU + 1, 0) : rv(RT_LMU + 4, 0),
-                        c2 = ot == 1 ? rv(RT_LMU + 2, 0) : rv(RT_LMU + 5, 0);
+                const T c0 = rvs(ot == 1, RT_LMU, RT_LMU + 3), c1 = rvs(ot == 1, RT_LMU + 1, RT_LMU + 4),
+                        c2 = rvs(ot == 1, RT_LMU + 2, RT_LMU + 5);
                 Lu = dv((c0 * (s_pll[k] - s_pll[kw]) + c1 * s_plt[kn]) + c2 * s_plt[k], Du);
                 g = g + Lu;
             }
@@ -946,9 +951,8 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             }
             if (a.mix) {
                 const int ot = (oe + OE(s_m[ks])) < 1 ? 1 : (oe + OE(s_m[ks]));
-                const T c0 = ot == 1 ? rv(RT_LMV, 0) : rv(RT_LMV + 3, 0),
-                        c1 = ot == 1 ? rv(RT_LMV + 1, 0) : rv(RT_LMV + 4, 0),
-                        c2 = ot == 1 ? rv(RT_LMV + 2, 0) : rv(RT_LMV + 5, 0);
+                const T c0 = rvs(ot == 1, RT_LMV, RT_LMV + 3), c1 = rvs(ot == 1, RT_LMV + 1, RT_LMV + 4),
+                        c2 = rvs(ot == 1, RT_LMV + 2, RT_LMV + 5);
                 Lv = dv((c0 * (s_plt[ke] - s_plt[k]) + c1 * s_pll[k]) + c2 * s_pll[ks], Dv);
                 g = g + Lv;
             }
