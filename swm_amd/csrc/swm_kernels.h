@@ -172,6 +172,9 @@ __device__ __forceinline__ T noland_w(int s) {
     return s <= 1 ? T(1) : (s == 2 ? T(0.5) : (s == 3 ? T(1) / T(3) : T(0.25)));
 }
 
+// x mod n for x in [-n, 2n)
+__device__ __forceinline__ int wrap1(int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); }
+
 __device__ __forceinline__ int wrap(int x, int n) {
     if (x < 0) x += n;
     if (x >= n) x -= n;
@@ -468,6 +471,9 @@ __device__ __forceinline__ void tile_load(const Dyn<T>& d, const int t, RG& R) {
 
     if (HIST) tile_load_hist<T, VAR, NTRC, TX, TY, DIAG>(d, t, R);
 
+    // tiles no larger than the grid: every halo and table index is within one
+    // period of it
+    const bool one = Nx >= TX && Ny >= TY;
     // ---- phase 1: state with a one-cell halo (periodic indices) into LDS.
     // Every global load of the block is issued before the first LDS store, so
     // one memory latency covers the whole tile and its per-row tables.
@@ -485,7 +491,14 @@ __device__ __forceinline__ void tile_load(const Dyn<T>& d, const int t, RG& R) {
         const int e = tid + q * NT;
         if (e < E) {
             const int lx = e % EW, ly = e / EW;
-            const int gi = wrap(i0 + lx - 1, Nx), gj = wrap(j0 + ly - 1, Ny);
+            int gi = i0 + lx - 1, gj = j0 + ly - 1;
+            if (one) {   // uniform: the halo is at most one period outside the grid
+                gi = wrap1(gi, Nx);
+                gj = wrap1(gj, Ny);
+            } else {
+                gi = wrap(gi, Nx);
+                gj = wrap(gj, Ny);
+            }
             const size_t off = (size_t)gj * P + gi;
             rm[q] = gld(a.mask, off);
             ru[q] = gld(su, off);
@@ -532,7 +545,8 @@ __device__ __forceinline__ void tile_load(const Dyn<T>& d, const int t, RG& R) {
         for (int q = 0; q < RTC::NQ; ++q) {
             const int x = tid + q * NT;
             if (x < RTC::R * RTC::K)
-                R.rt[q] = gld(a.rowtab, (size_t)wrap(j0 + x / RTC::K, Ny) * RT_N + x % RTC::K);
+                R.rt[q] = gld(a.rowtab, (size_t)(one ? wrap1(j0 + x / RTC::K, Ny) : wrap(j0 + x / RTC::K, Ny)) * RT_N +
+                                            x % RTC::K);
         }
     }
 }
