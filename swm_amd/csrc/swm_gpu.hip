@@ -1255,24 +1255,25 @@ int sor_prepare(swm_ctx* c, bool f32, int nblk) {
 // into chi_out, then the convergence test if sweep l+K-1 is a test sweep.
 template <typename T, int K>
 void sor_block_launch(swm_ctx* c, const void* chi_in, void* chi_out, void* dB, void* const* tabs, double omega,
-                      double tol, double bnorm, int64_t l, bool check) {
+                      double tol, double bnorm, int64_t l, bool check, bool ref_mode) {
     constexpr int TX = 64, TY = 16;
     const int ntx = (c->Nx + TX - 1) / TX;
     const int nb = ntx * ((c->Ny + TY - 1) / TY);
     swm::SorArgs<T> a{};
     a.Nx = c->Nx; a.Ny = c->Ny; a.P = c->P; a.color = 0; a.sweep = (int)l;
     a.chi = nullptr; a.B = (const T*)dB; a.mask = c->mask;
-    a.relax = nullptr; a.nrelax = 0; a.omega = omega;
+    a.relax = ref_mode ? (const double*)c->sor_relax : nullptr;
+    a.nrelax = c->sor_nrelax; a.omega = omega;
     a.c1t = (const T*)tabs[0]; a.c3t = (const T*)tabs[1]; a.c4t = (const T*)tabs[2]; a.c5t = (const T*)tabs[3]; a.c5r = (const T*)tabs[4];
     a.partial = c->sor_partial;
     a.done = c->sor_done;
-    a.residual_mode = 1;
+    a.residual_mode = ref_mode ? 0 : 1;
     a.accumulate = check ? 1 : 0;
     hipLaunchKernelGGL((swm::sor_block<T, TX, TY, K>), dim3(nb), dim3(256), 0, c->stream, a, (const T*)chi_in,
                        (T*)chi_out, ntx);
     if (check)
         hipLaunchKernelGGL(swm::sor_check, dim3(1), dim3(256), 0, c->stream, (const double*)c->sor_partial, nb,
-                           (int)(l + K - 1), tol, bnorm, 1, c->sor_done, c->sor_iters, c->sor_norm);
+                           (int)(l + K - 1), tol, bnorm, ref_mode ? 0 : 1, c->sor_done, c->sor_iters, c->sor_norm);
 }
 
 template <typename T>
@@ -1282,9 +1283,11 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
     const int batch = std::max(32, 4 * every);
     *host_done = 0;
     int64_t l = 1;
-    // Fixed-omega mode testing every 2 or 4 sweeps (or a multiple): K sweeps
-    // per blocked launch, chi alternating between dchi and the second buffer.
-    const int K = (ref_mode || !c->sor_blocked) ? 0 : (every % 4 == 0 ? 4 : (every % 2 == 0 ? 2 : 0));
+    // Blocked sweeps, chi alternating between dchi and the second buffer: K = 4
+    // or 2 sweeps per launch in fixed-omega mode when the test cadence is a
+    // multiple of it, else (and in the reference mode, which tests every sweep
+    // with its Chebyshev omega) one sweep per launch.
+    const int K = !c->sor_blocked ? 0 : (ref_mode ? 1 : (every % 4 == 0 ? 4 : (every % 2 == 0 ? 2 : 1)));
     if (K) {
         const int w = std::is_same<T, float>::value ? 1 : 0;
         void* cur = dchi;
@@ -1294,8 +1297,9 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
             const int64_t lend = std::min<int64_t>(nfull, l + batch - 1);
             for (; l + K - 1 <= lend; l += K) {
                 const bool check = ((l + K - 1) % every) == 0 || l + K - 1 == max_sweeps;
-                if (K == 4) sor_block_launch<T, 4>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check);
-                else sor_block_launch<T, 2>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check);
+                if (K == 4) sor_block_launch<T, 4>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode);
+                else if (K == 2) sor_block_launch<T, 2>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode);
+                else sor_block_launch<T, 1>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode);
                 std::swap(cur, alt);
             }
             HIPCHK(c, hipGetLastError());

@@ -1714,6 +1714,14 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
         const int color = hs & 1;
         const int lo = hs + 1;
         const bool tail = a.accumulate && hs >= 2 * K - 2;
+        // omega of this half-sweep: the reference's Chebyshev schedule (held
+        // once stationary) or the fixed omega; a.sweep is the launch's first sweep
+        T om = T(a.omega);
+        if (a.relax) {
+            int h = 2 * (a.sweep - 1) + hs;
+            if (h >= a.nrelax) h = a.nrelax - 1;
+            om = T(a.relax[h]);
+        }
         const int n = regular ? HH * (W / 2) : E;
         for (int q = tid; q < n; q += 256) {
             int lx, ly;
@@ -1740,10 +1748,12 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
             const T x = s_chi[e];
             const T lhs = (((c1 * s_chi[e + 1] + c2 * s_chi[e - 1]) + c3 * s_chi[e + W]) + c4 * s_chi[e - W]) + c5 * x;
             const T r = lhs - s_B[e];
-            const T res = dvr(T(a.omega) * r, c5, a.c5r[c5i]);
+            const T res = dvr(om * r, c5, a.c5r[c5i]);
             s_chi[e] = x - res;
-            if (tail && lx >= H && lx < H + TX && ly >= H && ly < H + TY && i0 + lx - H < Nx && j0 + ly - H < Ny)
-                acc += (double)r * (double)r;
+            if (tail && lx >= H && lx < H + TX && ly >= H && ly < H + TY && i0 + lx - H < Nx && j0 + ly - H < Ny) {
+                const double q = a.residual_mode ? (double)r : (double)res;
+                acc += q * q;
+            }
         }
         __syncthreads();
     }
