@@ -81,3 +81,23 @@ def test_blocked_converges_like_unblocked(gpu_lib, monkeypatch):
     assert ok_b and ok_u and it_b % 8 == 0 and abs(it_b - it_u) <= 8 and rr_b < 1e-9
     if it_b == it_u:
         assert same(chi_b, chi_u)
+
+
+@pytest.mark.parametrize("blocked", ["1", "0"])
+def test_reference_mode_one_sweep_blocks(gpu_lib, monkeypatch, blocked):
+    """main_ElSolv_SOR (Chebyshev omega per half-sweep, test after every sweep)
+    with one sweep per launch on LDS tiles, and with one colour per launch: the
+    oracle's chi bit for bit and its sweep count, on a grid of several tiles
+    with an odd row count."""
+    monkeypatch.setenv("SWM_GPU_SOR_BLOCK", blocked)
+    s = basin(130, 75, 0.06, seed=13)
+    o = oracle.Oracle(s)
+    B = rhs(o, s, 8)
+    chi_o, it_o, an_o, rc_o = o.sor(B, np.zeros(s.shape), epsilon=1e-4, first_guess=True)
+    g = gpu.SwmGpu(s)
+    chi_g, it_g, an_g, ok = g.sor(B, np.zeros(s.shape), epsilon=1e-4, first_guess=True)
+    assert ok and rc_o == 0 and it_g == it_o
+    assert same(chi_g, chi_o)
+    assert abs(an_g - an_o) <= 1e-12 * an_o
+    g.close()
+    o.close()
