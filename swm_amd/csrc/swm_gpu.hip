@@ -1201,8 +1201,8 @@ namespace {
 int sor_prepare(swm_ctx* c, bool f32, int nblk) {
     if (!c->sor_partial) {
         // partial sums of the colour launches (2 x nblk) or of the blocked
-        // sweeps (one per 64 x 16 tile)
-        const size_t nb = std::max<size_t>(2 * (size_t)nblk, (size_t)((c->Nx + 63) / 64) * ((c->Ny + 15) / 16));
+        // sweeps (one per 64 x 16 tile, two sets by launch parity)
+        const size_t nb = std::max<size_t>(2 * (size_t)nblk, 2 * (size_t)((c->Nx + 63) / 64) * ((c->Ny + 15) / 16));
         c->sor_partial = (double*)dalloc(c, sizeof(double) * nb);
         c->sor_done = (int*)dalloc(c, sizeof(int));
         c->sor_iters = (long long*)dalloc(c, sizeof(long long));
@@ -1255,7 +1255,7 @@ int sor_prepare(swm_ctx* c, bool f32, int nblk) {
 // into chi_out, then the convergence test if sweep l+K-1 is a test sweep.
 template <typename T, int K>
 void sor_block_launch(swm_ctx* c, const void* chi_in, void* chi_out, void* dB, void* const* tabs, double omega,
-                      double tol, double bnorm, int64_t l, bool check, bool ref_mode) {
+                      double tol, double bnorm, int64_t l, bool check, bool ref_mode, int parity, int64_t prev_sweep) {
     constexpr int TX = 64, TY = 16;
     const int ntx = (c->Nx + TX - 1) / TX;
     const int nb = ntx * ((c->Ny + TY - 1) / TY);
@@ -1265,15 +1265,21 @@ void sor_block_launch(swm_ctx* c, const void* chi_in, void* chi_out, void* dB, v
     a.relax = ref_mode ? (const double*)c->sor_relax : nullptr;
     a.nrelax = c->sor_nrelax; a.omega = omega;
     a.c1t = (const T*)tabs[0]; a.c3t = (const T*)tabs[1]; a.c4t = (const T*)tabs[2]; a.c5t = (const T*)tabs[3]; a.c5r = (const T*)tabs[4];
-    a.partial = c->sor_partial;
+    // partial sums alternate between two halves of the buffer by launch parity:
+    // block 0 of the next launch reads this launch's while its blocks write theirs
+    a.partial = c->sor_partial + (size_t)parity * nb;
     a.done = c->sor_done;
     a.residual_mode = ref_mode ? 0 : 1;
     a.accumulate = check ? 1 : 0;
+    a.prev_partial = c->sor_partial + (size_t)(parity ^ 1) * nb;
+    a.prev_check = prev_sweep > 0 ? 1 : 0;
+    a.prev_sweep = (int)prev_sweep;
+    a.eps = tol;
+    a.bnorm = bnorm;
+    a.iters = c->sor_iters;
+    a.norm_out = c->sor_norm;
     hipLaunchKernelGGL((swm::sor_block<T, TX, TY, K>), dim3(nb), dim3(256), 0, c->stream, a, (const T*)chi_in,
                        (T*)chi_out, ntx);
-    if (check)
-        hipLaunchKernelGGL(swm::sor_check, dim3(1), dim3(256), 0, c->stream, (const double*)c->sor_partial, nb,
-                           (int)(l + K - 1), tol, bnorm, ref_mode ? 0 : 1, c->sor_done, c->sor_iters, c->sor_norm);
 }
 
 template <typename T>
@@ -1289,22 +1295,42 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
     // with its Chebyshev omega) one sweep per launch.
     const int K = !c->sor_blocked ? 0 : (ref_mode ? 1 : (every % 4 == 0 ? 4 : (every % 2 == 0 ? 2 : 1)));
     if (K) {
+        // Launch n does sweeps n*K+1 .. (n+1)*K from one buffer into the other
+        // (dchi -> alt for even n); its convergence test runs in block 0 of
+        // launch n+1, or in a one-block check at the end of a batch.  After a
+        // passed test later launches return at once, so the converged chi is
+        // in the buffer of the launch whose sweep passed.
         const int w = std::is_same<T, float>::value ? 1 : 0;
         void* cur = dchi;
         void* alt = c->sor_alt[w];
+        const int ntx = (c->Nx + 63) / 64, nb = ntx * ((c->Ny + 15) / 16);
         const int64_t nfull = max_sweeps / K * K;
+        int64_t n = 0;
         while (l + K - 1 <= nfull && !*host_done) {
             const int64_t lend = std::min<int64_t>(nfull, l + batch - 1);
-            for (; l + K - 1 <= lend; l += K) {
+            int64_t pending = 0;   // sweep of the last launch's test not yet run (0: none)
+            for (; l + K - 1 <= lend; l += K, ++n) {
                 const bool check = ((l + K - 1) % every) == 0 || l + K - 1 == max_sweeps;
-                if (K == 4) sor_block_launch<T, 4>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode);
-                else if (K == 2) sor_block_launch<T, 2>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode);
-                else sor_block_launch<T, 1>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode);
+                const int par = (int)(n & 1);
+                if (K == 4) sor_block_launch<T, 4>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending);
+                else if (K == 2) sor_block_launch<T, 2>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending);
+                else sor_block_launch<T, 1>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending);
+                pending = check ? l + K - 1 : 0;
                 std::swap(cur, alt);
             }
+            if (pending)
+                hipLaunchKernelGGL(swm::sor_check, dim3(1), dim3(256), 0, c->stream,
+                                   (const double*)c->sor_partial + (size_t)((n - 1) & 1) * nb, nb, (int)pending, tol,
+                                   bnorm, ref_mode ? 0 : 1, c->sor_done, c->sor_iters, c->sor_norm);
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipMemcpyAsync(host_done, c->sor_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        if (*host_done) {   // the buffer of the launch whose test passed
+            long long it = 0;
+            HIPCHK(c, hipMemcpy(&it, c->sor_iters, sizeof(it), hipMemcpyDeviceToHost));
+            const int64_t nconv = it / K - 1;
+            cur = (nconv & 1) ? dchi : c->sor_alt[w];
         }
         if (cur != dchi)   // the result in the caller's buffer
             HIPCHK(c, hipMemcpyAsync(dchi, cur, (size_t)c->P * c->Ny * sizeof(T), hipMemcpyDeviceToDevice,

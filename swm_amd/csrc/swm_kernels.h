@@ -1602,10 +1602,40 @@ struct SorArgs {
     const T* c5t;               // [Ny][3][2][2] c5 by (ou(i+1)+ou(i), ov(j+1), ov(j))
     const T* c5r;               // 1 / c5t, correctly rounded (dvr)
     double* partial;            // per-block partial sums of res^2 (or r^2)
-    const int* done;            // set once converged: later sweeps return at once
+    int* done;                  // set once converged: later sweeps return at once
     int residual_mode;          // 0: sum res^2 (reference), 1: sum (L chi - B)^2
     int accumulate;             // 0: no convergence test after this sweep (no partial sums)
+    // blocked sweeps: block 0 runs the previous launch's convergence test
+    const double* prev_partial; // that launch's partial sums (gridDim.x of them)
+    int prev_check, prev_sweep; // whether it had a test, and the sweep tested
+    double eps, bnorm;
+    long long* iters;
+    double* norm_out;
 };
+
+// The convergence test of one sweep, by the 256 threads of one block: sum the
+// n partials in a fixed order, take the norm (mode 0: sqrt(sum), the
+// reference's; mode 1: sqrt(sum) / bnorm), record it and the sweep, and set
+// *done when it is below eps.  Returns that outcome to every thread.
+__device__ __forceinline__ bool sor_test(const double* partial, int n, int sweep, double eps, double bnorm, int mode,
+                                         int* done, long long* iters, double* norm_out, double* red, int* flag) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < n; b += 256) acc += partial[b];
+    for (int s = 32; s >= 1; s >>= 1) acc += __shfl_xor(acc, s, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double tot = ((red[0] + red[1]) + red[2]) + red[3];
+        double nrm = sqrt(tot);
+        if (mode == 1) nrm = bnorm > 0.0 ? nrm / bnorm : nrm;
+        *norm_out = nrm;
+        *iters = sweep;
+        *flag = nrm < eps;
+        if (nrm < eps) *done = 1;
+    }
+    __syncthreads();
+    return *flag != 0;
+}
 
 // Each thread updates one cell of the colour; row j holds the colour's cells
 // at i = (j + colour + 1) % 2 + 2k (colour 0 = (i+j) even in 1-based indices).
@@ -1683,13 +1713,15 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
     const int Nx = a.Nx, Ny = a.Ny, P = a.P;
     const int i0 = (blockIdx.x % ntx) * TX, j0 = (blockIdx.x / ntx) * TY;
     const int tid = threadIdx.x;
-    if (*a.done) {   // converged: carry the output cells over to the other buffer
-        for (int o = tid; o < TX * TY; o += 256) {
-            const int i = i0 + o % TX, j = j0 + o / TX;
-            if (i < Nx && j < Ny) chi_out[(size_t)j * P + i] = chi_in[(size_t)j * P + i];
-        }
+    // Block 0 tests the previous launch's sweep while the others sweep on; once
+    // a test has passed, later launches return at once and the converged chi
+    // stays in the buffer that launch wrote (the host picks it by the sweep).
+    __shared__ int conv;
+    if (a.prev_check && blockIdx.x == 0 && !*a.done &&
+        sor_test(a.prev_partial, (int)gridDim.x, a.prev_sweep, a.eps, a.bnorm, a.residual_mode ? 1 : 0, a.done,
+                 a.iters, a.norm_out, red, &conv))
         return;
-    }
+    if (*a.done) return;
     for (int e = tid; e < E; e += 256) {
         const int lx = e % W, ly = e / W;
         const int gi = wrap(i0 - H + lx, Nx), gj = wrap(j0 - H + ly, Ny);
@@ -1775,20 +1807,9 @@ __global__ void __launch_bounds__(256)
 sor_check(const double* partial, int nblk, int sweep, double eps, double bnorm, int mode,
           int* done, long long* iters, double* norm_out) {
     __shared__ double red[4];
+    __shared__ int conv;
     if (*done) return;
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += 256) acc += partial[b];
-    for (int s = 32; s >= 1; s >>= 1) acc += __shfl_xor(acc, s, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const double tot = ((red[0] + red[1]) + red[2]) + red[3];
-        double nrm = sqrt(tot);
-        if (mode == 1) nrm = bnorm > 0.0 ? nrm / bnorm : nrm;
-        *norm_out = nrm;
-        *iters = sweep;
-        if (nrm < eps) *done = 1;
-    }
+    sor_test(partial, nblk, sweep, eps, bnorm, mode, done, iters, norm_out, red, &conv);
 }
 
 // computeDivergence(u, v, div) (src/calc_lib.f90:607-622), evaluated in the
