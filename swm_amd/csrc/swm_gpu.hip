@@ -1286,7 +1286,11 @@ template <typename T>
 int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mode, double omega, double tol,
                       double bnorm, int64_t max_sweeps, int nblk, int* host_done) {
     const int every = ref_mode ? 1 : c->sor_every;   // the reference tests after every sweep
-    const int batch = std::max(32, 4 * every);
+    // sweeps enqueued between reads of the convergence flag: 32 at first (a
+    // warm-started solve stops early without queueing many idle launches),
+    // doubling up to 256 for long solves (fewer host round trips)
+    int64_t batch = std::max(32, 4 * every);
+    const int64_t batch_max = std::max(256, 4 * every);
     *host_done = 0;
     int64_t l = 1;
     // Blocked sweeps, chi alternating between dchi and the second buffer: K = 4
@@ -1308,6 +1312,7 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
         int64_t n = 0;
         while (l + K - 1 <= nfull && !*host_done) {
             const int64_t lend = std::min<int64_t>(nfull, l + batch - 1);
+            batch = std::min(2 * batch, batch_max);
             int64_t pending = 0;   // sweep of the last launch's test not yet run (0: none)
             for (; l + K - 1 <= lend; l += K, ++n) {
                 const bool check = ((l + K - 1) % every) == 0 || l + K - 1 == max_sweeps;
@@ -1339,6 +1344,7 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
     }
     while (l <= max_sweeps && !*host_done) {
         const int64_t lend = std::min<int64_t>(max_sweeps, l + batch - 1);
+        batch = std::min(2 * batch, batch_max);
         for (; l <= lend; ++l) {
             const bool check = (l % every) == 0 || l == max_sweeps;
             for (int color = 0; color < 2; ++color) {
