@@ -1693,14 +1693,15 @@ __global__ void __launch_bounds__(256) sor_sweep(const SorArgs<T> a) {
     if (threadIdx.x == 0) a.partial[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-// K fixed-omega sweeps in one launch on a TX x TY tile held in LDS with a
-// 2K-cell halo (read from chi_in, written to chi_out: blocks read their halos
-// while neighbours write).  Half sweep hs updates the colour-(hs % 2) cells at
-// least hs + 1 cells inside the staged block, whose dependency cones stay
-// inside it, so the K sweeps give each output cell exactly the values of K
-// global red-black sweeps (sor_sweep, same arithmetic).  With accumulate the
-// residuals r of the last sweep's updates of the output cells are summed in
-// FP64 in a fixed order into partial[block].
+// K sweeps in one launch (fixed omega, or the reference's Chebyshev omega per
+// half sweep) on a TX x TY tile held in LDS with a 2K-cell halo (read from
+// chi_in, written to chi_out: blocks read their halos while neighbours write).
+// Half sweep hs updates the colour-(hs % 2) cells at least hs + 1 cells inside
+// the staged block, whose dependency cones stay inside it, so the K sweeps give
+// each output cell exactly the values of K global red-black sweeps (sor_sweep,
+// same arithmetic).  With accumulate the last sweep's residuals of the output
+// cells (r, or the reference's res) are summed in FP64 in a fixed order into
+// partial[block]; block 0 first runs the previous launch's convergence test.
 template <typename T, int TX, int TY, int K>
 __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __restrict__ chi_in,
                                                  T* __restrict__ chi_out, int ntx) {
