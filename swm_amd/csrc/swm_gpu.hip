@@ -257,6 +257,25 @@ int fail(swm_ctx* c, const std::string& msg, int code = 2) {
             return fail((ctx), std::string(#call) + ": " + hipGetErrorString(e_), 3);          \
     } while (0)
 
+// Every copy and fill runs on the context's own stream and is waited for.
+// The stream is non-blocking, so work on the legacy null stream (plain
+// hipMemcpy / hipMemset) would not be ordered with the kernels on it.
+hipError_t copy_sync(swm_ctx* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
+}
+
+hipError_t copy2d_sync(swm_ctx* c, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                       size_t rows, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, kind, c->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
+}
+
+hipError_t zero_sync(swm_ctx* c, void* dst, size_t bytes) {
+    hipError_t e = hipMemsetAsync(dst, 0, bytes, c->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
+}
+
 void* dalloc(swm_ctx* c, size_t bytes) {
     void* p = nullptr;
     if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) return nullptr;
@@ -277,9 +296,9 @@ void* upload_vec(swm_ctx* c, const std::vector<double>& h) {
     if (!p) return nullptr;
     if (c->fp32) {
         std::vector<float> f(h.begin(), h.end());
-        if (hipMemcpy(p, f.data(), f.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        if (copy_sync(c, p, f.data(), f.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
     } else {
-        if (hipMemcpy(p, h.data(), h.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        if (copy_sync(c, p, h.data(), h.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
     }
     return p;
 }
@@ -290,9 +309,9 @@ int upload_grid(swm_ctx* c, void* dst, const double* src) {
     if (c->fp32) {
         std::vector<float> f(n);
         for (size_t k = 0; k < n; ++k) f[k] = (float)src[k];
-        HIPCHK(c, hipMemcpy2D(dst, c->P * 4, f.data(), c->Nx * 4, c->Nx * 4, c->Ny, hipMemcpyHostToDevice));
+        HIPCHK(c, copy2d_sync(c, dst, c->P * 4, f.data(), c->Nx * 4, c->Nx * 4, c->Ny, hipMemcpyHostToDevice));
     } else {
-        HIPCHK(c, hipMemcpy2D(dst, c->P * 8, src, c->Nx * 8, c->Nx * 8, c->Ny, hipMemcpyHostToDevice));
+        HIPCHK(c, copy2d_sync(c, dst, c->P * 8, src, c->Nx * 8, c->Nx * 8, c->Ny, hipMemcpyHostToDevice));
     }
     return 0;
 }
@@ -301,17 +320,17 @@ int download_grid(swm_ctx* c, double* dst, const void* src) {
     const size_t n = (size_t)c->Nx * c->Ny;
     if (c->fp32) {
         std::vector<float> f(n);
-        HIPCHK(c, hipMemcpy2D(f.data(), c->Nx * 4, src, c->P * 4, c->Nx * 4, c->Ny, hipMemcpyDeviceToHost));
+        HIPCHK(c, copy2d_sync(c, f.data(), c->Nx * 4, src, c->P * 4, c->Nx * 4, c->Ny, hipMemcpyDeviceToHost));
         for (size_t k = 0; k < n; ++k) dst[k] = f[k];
     } else {
-        HIPCHK(c, hipMemcpy2D(dst, c->Nx * 8, src, c->P * 8, c->Nx * 8, c->Ny, hipMemcpyDeviceToHost));
+        HIPCHK(c, copy2d_sync(c, dst, c->Nx * 8, src, c->P * 8, c->Nx * 8, c->Ny, hipMemcpyDeviceToHost));
     }
     return 0;
 }
 
 void* alloc_grid(swm_ctx* c) {
     void* p = dalloc(c, (size_t)c->Ny * c->P * c->esz);
-    if (p && hipMemset(p, 0, (size_t)c->Ny * c->P * c->esz) != hipSuccess) return nullptr;
+    if (p && zero_sync(c, p, (size_t)c->Ny * c->P * c->esz) != hipSuccess) return nullptr;
     return p;
 }
 
@@ -513,8 +532,7 @@ int write_statics_t(swm_ctx* c) {
     if (!c->statics && !(c->statics = dalloc(c, sizeof(Statics<T>)))) return fail(c, "out of device memory", 3);
     if (!c->rowtab && !(c->rowtab = dalloc(c, (size_t)c->Ny * swm::RT_N * sizeof(T)))) return fail(c, "out of device memory", 3);
     a.rowtab = (const T*)c->rowtab;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(c->statics, &a, sizeof(a), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_sync(c, c->statics, &a, sizeof(a), hipMemcpyHostToDevice));
     const int nrt = c->Ny * swm::RT_N;
     swm::pack_rowtab<T><<<(nrt + 255) / 256, 256, 0, c->stream>>>((const Statics<T>*)c->statics, (T*)c->rowtab, c->Ny);
     HIPCHK(c, hipGetLastError());
@@ -747,36 +765,95 @@ void drop_graphs(swm_ctx* c) {
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
 }
 
+// Capture the six steady-state steps that start from ping-pong level `cur`
+// and ring slot `ring` into c->graphs[cur * 3 + ring].  Capturing records the
+// launches without running them, so the host-side step state is put back.
+template <typename T>
+int capture_graph(swm_ctx* c, int cur, int ring) {
+    const int key = cur * 3 + ring;
+    if (c->graphs[key]) return 0;
+    HipLock lock(g_hip_mu);
+    const int cur0 = c->cur, ring0 = c->ring;
+    const int64_t itt0 = c->itt;
+    std::vector<double> w0;
+    for (auto& av : c->avgs) w0.push_back(av.weight);
+    std::vector<bool> cb0;
+    for (auto& tr : c->trc) {
+        cb0.push_back(tr.cbuf_valid);
+        tr.cbuf_valid = false;    // steady steps read no host C buffer
+    }
+    c->cur = cur;
+    c->ring = ring;
+    if (c->itt < 2) c->itt = 2;   // steady steps use the full AB3 coefficients
+    hipGraph_t g = nullptr;
+    int rc = 0;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        for (int k = 0; k < 6 && !rc; ++k) rc = step_one<T>(c);
+        e = hipStreamEndCapture(c->stream, &g);
+    }
+    c->cur = cur0;
+    c->ring = ring0;
+    c->itt = itt0;
+    for (size_t q = 0; q < w0.size(); ++q) c->avgs[q].weight = w0[q];
+    for (size_t q = 0; q < cb0.size(); ++q) c->trc[q].cbuf_valid = cb0[q];
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(c, std::string("graph capture: ") + hipGetErrorString(e), 3);
+    e = hipGraphInstantiate(&c->graphs[key], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        c->graphs[key] = nullptr;
+        return fail(c, std::string("graph instantiate: ") + hipGetErrorString(e), 3);
+    }
+    // move the graph's one-time device set-up out of the first replay
+    HIPCHK(c, hipGraphUpload(c->graphs[key], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 // Steady state (itt >= 3, no first-step tracer buffer, no oscillating or
 // time-interpolated stream, whose per-step arguments change every step):
 // the launch arguments repeat with the period 6 of the ping-pong x ring roles,
 // so six steps are captured once per phase and replayed as one graph.
+bool steady_ok(const swm_ctx* c) { return c->use_graph && !c->has_osc && c->streams.empty(); }
+
+// Build, before any step is enqueued, every graph that step_many(nsteps) will
+// replay, so that the host-side capture and instantiation stay outside the
+// device-time window of swm_gpu_step.
+template <typename T>
+int prepare_graphs(swm_ctx* c, int64_t nsteps) {
+    if (!steady_ok(c)) return 0;
+    int cur = c->cur, ring = c->ring;
+    int64_t itt = c->itt, done = 0;
+    bool cbuf = false;
+    for (auto& tr : c->trc) cbuf = cbuf || tr.cbuf_valid;
+    while (done < nsteps) {
+        if (!cbuf && itt >= 2 && nsteps - done >= 6) {
+            if (int rc = capture_graph<T>(c, cur, ring)) return rc;
+            return 0;   // a six-step cycle leaves (cur, ring) unchanged: one graph serves the rest
+        }
+        itt += 1;
+        done += 1;
+        cur = 1 - cur;
+        ring = (ring + 1) % 3;
+        cbuf = false;
+    }
+    return 0;
+}
+
 template <typename T>
 int step_many(swm_ctx* c, int64_t nsteps) {
     int64_t done = 0;
     while (done < nsteps) {
         bool cbuf = false;
         for (auto& tr : c->trc) cbuf = cbuf || tr.cbuf_valid;
-        const bool steady = c->use_graph && !c->has_osc && c->streams.empty() && !cbuf && c->itt >= 2;
+        const bool steady = steady_ok(c) && !cbuf && c->itt >= 2;
         if (steady && nsteps - done >= 6) {
             const int key = c->cur * 3 + c->ring;
-            if (!c->graphs[key]) {
-                HipLock lock(g_hip_mu);
-                std::vector<double> w0;
-                for (auto& av : c->avgs) w0.push_back(av.weight);
-                hipGraph_t g = nullptr;
-                HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                int rc = 0;
-                for (int k = 0; k < 6 && !rc; ++k) rc = step_one<T>(c);
-                hipError_t e = hipStreamEndCapture(c->stream, &g);
-                if (rc) return rc;
-                if (e != hipSuccess) return fail(c, std::string("graph capture: ") + hipGetErrorString(e), 3);
-                e = hipGraphInstantiate(&c->graphs[key], g, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(g);
-                if (e != hipSuccess) return fail(c, std::string("graph instantiate: ") + hipGetErrorString(e), 3);
-                c->itt -= 6;   // capture recorded the six steps without running them
-                for (size_t q = 0; q < w0.size(); ++q) c->avgs[q].weight = w0[q];
-            }
+            if (int rc = capture_graph<T>(c, c->cur, c->ring)) return rc;
             HIPCHK(c, hipGraphLaunch(c->graphs[key], c->stream));
             c->itt += 6;
             done += 6;
@@ -823,10 +900,10 @@ int alloc_state(swm_ctx* c, int ntrc) {
     const int planes = swm::PL_TRC + swm::PL_PER_TRC * ntrc;
     void* p = dalloc(c, (size_t)planes * c->plane * c->esz);
     if (!p) return fail(c, "out of device memory", 3);
-    HIPCHK(c, hipMemset(p, 0, (size_t)planes * c->plane * c->esz));
+    HIPCHK(c, zero_sync(c, p, (size_t)planes * c->plane * c->esz));
     if (c->state) {
         const int old = swm::PL_TRC + swm::PL_PER_TRC * (ntrc - 1);
-        HIPCHK(c, hipMemcpy(p, c->state, (size_t)old * c->plane * c->esz, hipMemcpyDeviceToDevice));
+        HIPCHK(c, copy_sync(c, p, c->state, (size_t)old * c->plane * c->esz, hipMemcpyDeviceToDevice));
         dfree(c, c->state);
     }
     c->state = p;
@@ -938,7 +1015,7 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
             }
         c->mask = (uint8_t*)dalloc(c, m.size());
         if (!c->mask) return fail(c, "out of device memory", 3);
-        HIPCHK(c, hipMemcpy(c->mask, m.data(), m.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, copy_sync(c, c->mask, m.data(), m.size(), hipMemcpyHostToDevice));
     }
     const int8_t *mu = c->hmask[1].data(), *mv = c->hmask[2].data(), *me = c->hmask[3].data();
     int rc = 0;
@@ -1053,6 +1130,7 @@ int swm_gpu_step(swm_ctx* c, int64_t nsteps) {
     if (!c) return 1;
     if (c->batch) return fail(c, "context is attached to a batch; it advances through swm_gpu_batch_step", 1);
     if (nsteps < 0) return fail(c, "nsteps must be >= 0", 1);
+    if (int rc = c->fp32 ? prepare_graphs<float>(c, nsteps) : prepare_graphs<double>(c, nsteps)) return rc;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int rc = c->fp32 ? step_many<float>(c, nsteps) : step_many<double>(c, nsteps);
     if (rc) return rc;
@@ -1132,11 +1210,11 @@ int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_o
         const size_t n = (size_t)c->Nx * c->Ny;
         if (w == 1) {
             std::vector<float> f(n);
-            HIPCHK(c, hipMemcpy2D(f.data(), c->Nx * 4, c->sor_buf[1][0], c->P * 4, c->Nx * 4, c->Ny, hipMemcpyDeviceToHost));
+            HIPCHK(c, copy2d_sync(c, f.data(), c->Nx * 4, c->sor_buf[1][0], c->P * 4, c->Nx * 4, c->Ny, hipMemcpyDeviceToHost));
             for (size_t k = 0; k < n; ++k) host_out[k] = f[k];
         } else {
-            HIPCHK(c, hipMemcpy2D(host_out, c->Nx * 8, c->sor_buf[0][0], c->P * 8, c->Nx * 8, c->Ny,
-                                  hipMemcpyDeviceToHost));
+            HIPCHK(c, copy2d_sync(c, host_out, c->Nx * 8, c->sor_buf[0][0], c->P * 8, c->Nx * 8, c->Ny,
+                                   hipMemcpyDeviceToHost));
         }
         return 0;
     }
@@ -1226,10 +1304,10 @@ int sor_prepare(swm_ctx* c, bool f32, int nblk) {
             // tables were uploaded in the context precision; convert on the host
             std::vector<double> h(sz[q]);
             std::vector<float> f(sz[q]);
-            HIPCHK(c, hipMemcpy(h.data(), c->sor_tab[q], sz[q] * 8, hipMemcpyDeviceToHost));
+            HIPCHK(c, copy_sync(c, h.data(), c->sor_tab[q], sz[q] * 8, hipMemcpyDeviceToHost));
             for (size_t k = 0; k < sz[q]; ++k) f[k] = (float)h[k];
             if (!(c->sor_ftab[w][q] = dalloc(c, sz[q] * 4))) return fail(c, "out of device memory", 3);
-            HIPCHK(c, hipMemcpy(c->sor_ftab[w][q], f.data(), sz[q] * 4, hipMemcpyHostToDevice));
+            HIPCHK(c, copy_sync(c, c->sor_ftab[w][q], f.data(), sz[q] * 4, hipMemcpyHostToDevice));
         }
     }
     if (!c->sor_relax) {
@@ -1245,7 +1323,7 @@ int sor_prepare(swm_ctx* c, bool f32, int nblk) {
             wv = nw;
         }
         if (!(c->sor_relax = dalloc(c, relax.size() * 8))) return fail(c, "out of device memory", 3);
-        HIPCHK(c, hipMemcpy(c->sor_relax, relax.data(), relax.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, copy_sync(c, c->sor_relax, relax.data(), relax.size() * 8, hipMemcpyHostToDevice));
         c->sor_nrelax = (int)relax.size();
     }
     return 0;
@@ -1333,7 +1411,7 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
         }
         if (*host_done) {   // the buffer of the launch whose test passed
             long long it = 0;
-            HIPCHK(c, hipMemcpy(&it, c->sor_iters, sizeof(it), hipMemcpyDeviceToHost));
+            HIPCHK(c, copy_sync(c, &it, c->sor_iters, sizeof(it), hipMemcpyDeviceToHost));
             const int64_t nconv = it / K - 1;
             cur = (nconv & 1) ? dchi : c->sor_alt[w];
         }
@@ -1388,11 +1466,11 @@ int sor_common(swm_ctx* c, const double* B, double* chi, int ref_mode, double om
     if (f32) {
         std::vector<float> fc(n), fb(n);
         for (size_t k = 0; k < n; ++k) { fc[k] = (float)chi[k]; fb[k] = (float)B[k]; }
-        HIPCHK(c, hipMemcpy2D(dchi, pitch_b, fc.data(), Nx * 4, Nx * 4, Ny, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy2D(dB, pitch_b, fb.data(), Nx * 4, Nx * 4, Ny, hipMemcpyHostToDevice));
+        HIPCHK(c, copy2d_sync(c, dchi, pitch_b, fc.data(), Nx * 4, Nx * 4, Ny, hipMemcpyHostToDevice));
+        HIPCHK(c, copy2d_sync(c, dB, pitch_b, fb.data(), Nx * 4, Nx * 4, Ny, hipMemcpyHostToDevice));
     } else {
-        HIPCHK(c, hipMemcpy2D(dchi, pitch_b, chi, Nx * 8, Nx * 8, Ny, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy2D(dB, pitch_b, B, Nx * 8, Nx * 8, Ny, hipMemcpyHostToDevice));
+        HIPCHK(c, copy2d_sync(c, dchi, pitch_b, chi, Nx * 8, Nx * 8, Ny, hipMemcpyHostToDevice));
+        HIPCHK(c, copy2d_sync(c, dB, pitch_b, B, Nx * 8, Nx * 8, Ny, hipMemcpyHostToDevice));
     }
     // ||B|| over ocean eta points, in FP64 whatever the sweep precision
     double bnorm = 0.0;
@@ -1412,14 +1490,14 @@ int sor_common(swm_ctx* c, const double* B, double* chi, int ref_mode, double om
     if (rc) return rc;
     long long it = 0;
     double nrm = 0.0;
-    HIPCHK(c, hipMemcpy(&it, c->sor_iters, sizeof(it), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(&nrm, c->sor_norm, sizeof(nrm), hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c, &it, c->sor_iters, sizeof(it), hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c, &nrm, c->sor_norm, sizeof(nrm), hipMemcpyDeviceToHost));
     if (f32) {
         std::vector<float> fc(n);
-        HIPCHK(c, hipMemcpy2D(fc.data(), Nx * 4, dchi, pitch_b, Nx * 4, Ny, hipMemcpyDeviceToHost));
+        HIPCHK(c, copy2d_sync(c, fc.data(), Nx * 4, dchi, pitch_b, Nx * 4, Ny, hipMemcpyDeviceToHost));
         for (size_t k = 0; k < n; ++k) chi[k] = fc[k];
     } else {
-        HIPCHK(c, hipMemcpy2D(chi, Nx * 8, dchi, pitch_b, Nx * 8, Ny, hipMemcpyDeviceToHost));
+        HIPCHK(c, copy2d_sync(c, chi, Nx * 8, dchi, pitch_b, Nx * 8, Ny, hipMemcpyDeviceToHost));
     }
     *iters = it;
     *norm = nrm;
@@ -1468,8 +1546,8 @@ int velpot_t(swm_ctx* c, double omega, double rel_tol, int64_t max_sweeps, bool 
         return rc;
     long long it = 0;
     double nrm = 0.0;
-    HIPCHK(c, hipMemcpy(&it, c->sor_iters, sizeof(it), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(&nrm, c->sor_norm, sizeof(nrm), hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c, &it, c->sor_iters, sizeof(it), hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c, &nrm, c->sor_norm, sizeof(nrm), hipMemcpyDeviceToHost));
     *iters = it;
     *rel_res = nrm;
     c->chi_valid[w] = true;
@@ -1534,7 +1612,7 @@ int swm_gpu_add_forcing_stream(swm_ctx* c, int32_t kind, int32_t component, int3
     st.rec = dalloc(c, (size_t)nrec * c->plane * c->esz);
     if (!st.rec) return fail(c, "out of device memory for the stream records", 3);
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemset(st.rec, 0, (size_t)nrec * c->plane * c->esz));
+    HIPCHK(c, zero_sync(c, st.rec, (size_t)nrec * c->plane * c->esz));
     for (int32_t r = 0; r < nrec; ++r) {
         int rc = upload_grid(c, (char*)st.rec + (size_t)r * c->plane * c->esz, records + (size_t)r * c->Nx * c->Ny);
         if (rc) return rc;
@@ -1639,7 +1717,7 @@ int swm_gpu_avg_read(swm_ctx* c, int32_t avg_id, double* host_out, double* weigh
     swm_ctx::Avg& av = c->avgs[avg_id];
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const size_t n = (size_t)c->Nx * c->Ny;
-    HIPCHK(c, hipMemcpy2D(host_out, c->Nx * 8, av.buf, c->P * 8, c->Nx * 8, c->Ny, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy2d_sync(c, host_out, c->Nx * 8, av.buf, c->P * 8, c->Nx * 8, c->Ny, hipMemcpyDeviceToHost));
     // task%buffer = task%buffer / task%bufferCount
     for (size_t k = 0; k < n; ++k) host_out[k] = host_out[k] / av.weight;
     if (weight) *weight = av.weight;
@@ -1704,7 +1782,7 @@ int swm_gpu_sums(swm_ctx* c, int32_t tracer_id, double* out) {
         c->sums_buf = (double*)dalloc(c, sizeof(double) * (6 * 1024 + 3));
         c->ce64 = (double*)dalloc(c, sizeof(double) * c->Ny);
         if (!c->sums_buf || !c->ce64) return fail(c, "out of device memory", 3);
-        HIPCHK(c, hipMemcpy(c->ce64, c->ce_host.data(), sizeof(double) * c->Ny, hipMemcpyHostToDevice));
+        HIPCHK(c, copy_sync(c, c->ce64, c->ce_host.data(), sizeof(double) * c->Ny, hipMemcpyHostToDevice));
     }
     const int fn = c->d.variant == SWM_FULLY_NONLINEAR;
     const int ch_plane = tracer_id >= 0 ? swm::PL_TRC + swm::PL_PER_TRC * tracer_id + c->cur : -1;
@@ -1803,9 +1881,11 @@ void batch_drop_graph(swm_batch* b) {
 int batch_pull(swm_batch* b) {
     if (!b->on_device) return 0;
     BHIPCHK(b, hipStreamSynchronize(b->stream));
-    if (!b->hstep.empty())
-        BHIPCHK(b, hipMemcpy(b->hstep.data(), b->dstep, b->hstep.size() * sizeof(swm::MemberStep),
-                             hipMemcpyDeviceToHost));
+    if (!b->hstep.empty()) {
+        BHIPCHK(b, hipMemcpyAsync(b->hstep.data(), b->dstep, b->hstep.size() * sizeof(swm::MemberStep),
+                                  hipMemcpyDeviceToHost, b->stream));
+        BHIPCHK(b, hipStreamSynchronize(b->stream));
+    }
     b->on_device = false;
     for (size_t k = 0; k < b->mem.size(); ++k) {
         swm_ctx* c = b->mem[k];

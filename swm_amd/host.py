@@ -275,14 +275,21 @@ class Setup:
 # swm_vars / computeD aliasing for the linear variants
 # ---------------------------------------------------------------------------
 
-def set_mean_state(s: Setup, psi_bs: np.ndarray) -> None:
+def set_mean_state(s: Setup, psi_bs: np.ndarray, mask_psi: bool = False) -> None:
     """LINEARISED_MEAN_STATE basic state from its streamfunction on the H grid
     (SWM_timestep_init, swm_timestep_module.f90:76-87): u_bs = evSF_zonal,
     v_bs = evSF_meridional (calc_lib.f90:840-880), zeta_bs = laplacian2D
     (calc_lib.f90:1170-1206, the `vorticity` of a scalar).  Evaluated with the
     grid thicknesses as initDomain leaves them, i.e. before
-    apply_linear_D_alias, as the reference computes it before computeD."""
+    apply_linear_D_alias, as the reference computes it before computeD.
+
+    mask_psi multiplies the streamfunction by the H-grid ocean mask first, the
+    o_H Psi of evSF_zonal's documented formula (calc_lib.f90:832); the code
+    itself (calc_lib.f90:840-850, 1176-1205) uses Psi as read, which is the
+    default."""
     psi = np.asarray(psi_bs, dtype=np.float64)
+    if mask_psi:
+        psi = psi * s.grids["H"].ocean
     A, dL, dT = s.A, s.dLambda, s.dTheta
     gH, gu, gv = s.grids["H"], s.grids["u"], s.grids["v"]
     oH = gH.ocean.astype(np.int64)

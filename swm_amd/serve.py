@@ -48,6 +48,7 @@ class Request:
     t_admit: float = math.nan
     t_done: float = math.nan
     result: Optional[dict] = None
+    max_batch: int = 0        # most members that shared one launch with this request
 
     @property
     def cells(self) -> int:
@@ -120,6 +121,7 @@ class Server:
         self.used = 0.0
         self.cv = threading.Condition()
         self.chunks = 0
+        self.max_batch = 0        # most members advanced by one launch
         self.gpu_ms = 0.0
 
     def _producer(self, reqs, ready: "queue.Queue", t0: float, realtime: bool, stop: threading.Event):
@@ -188,6 +190,9 @@ class Server:
                     continue
                 batch.step(self.chunk)
                 self.chunks += 1
+                self.max_batch = max(self.max_batch, len(active))
+                for r, _, _ in active.values():
+                    r.max_batch = max(r.max_batch, len(active))
                 self.gpu_ms += batch.last_ms()
                 now = time.perf_counter() - t0
                 for key in list(active):
@@ -223,6 +228,6 @@ class Server:
         lat = np.array([r.latency for r in reqs])
         return {"requests": len(reqs), "wall_s": wall, "cell_steps": cell_steps,
                 "cell_updates_per_s": cell_steps / wall if wall > 0 else 0.0,
-                "gpu_busy_s": self.gpu_ms * 1e-3, "chunks": self.chunks,
+                "gpu_busy_s": self.gpu_ms * 1e-3, "chunks": self.chunks, "max_members_per_launch": self.max_batch,
                 "latency_p50_s": float(np.percentile(lat, 50)), "latency_p95_s": float(np.percentile(lat, 95)),
                 "latency_p99_s": float(np.percentile(lat, 99)), "latency_max_s": float(lat.max())}

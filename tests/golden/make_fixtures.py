@@ -90,6 +90,18 @@ def main():
         snap["time"] = t[[1, 5, 10]]
     np.savez_compressed(os.path.join(HERE, "diaglist_psi.npz"), H=H3, H_missing=np.float32(Ha3["missing_value"]),
                         params=params("test_diaglist"), **snap)
+    # test_diaglist: recorded state and derived register fields at three of the
+    # eleven snapshots, and the basic-state streamfunction input
+    der = {}
+    for fname, var in (("u_out", "u"), ("v_out", "v"), ("eta_out", "eta"), ("relvort", "swm_relvort"),
+                       ("potvort", "swm_pot"), ("edens", "swm_edens"), ("mu", "swm_mu"), ("mv", "swm_mv"),
+                       ("d", "swm_d")):
+        d, a, t = read_var(os.path.join(exp, f"output/reference_000000000001_{fname}.nc"), var)
+        der[var] = d[[1, 5, 10]]
+        der["missing"] = a["missing_value"]
+        der["time"] = t[[1, 5, 10]]
+    bs, _, _ = read_var(os.path.join(exp, "input/bs_sc.nc"), "PSI")
+    np.savez_compressed(os.path.join(HERE, "diaglist_derived.npz"), psi_bs=bs.astype(np.float64), **der)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

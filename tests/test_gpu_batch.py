@@ -96,3 +96,73 @@ def test_batch_rejects_mixed_precision(gpu_lib):
         assert a.itt == 3
     a.close()
     f.close()
+
+
+@pytest.mark.parametrize("precision", [gpu.FP32, gpu.FP64])
+def test_batch_tune_warmup_close_equals_alone(gpu_lib, precision):
+    """The ensemble benchmark's sequence: attach, tune (snapshot, timed runs of
+    each tile shape, restore), warm-up, timed steps, close.  After the batch is
+    closed every member equals the same member stepped alone, bit for bit, and
+    its FP64 sums equal the alone member's (volume and energy finite; the
+    tracer inventory is NaN once the land cells' 0/0 coefficients have made CH
+    NaN there, as in the reference)."""
+    setups = [configs.ensemble_member(seed=31 + k, Nx=nx, Ny=ny, H0=h, Ah=ah, tau0=t, tracer=tr, nsteps=40)
+              for k, (nx, ny, h, ah, t, tr) in enumerate([(128, 256, 900.0, 60.0, 0.1, True),
+                                                          (256, 128, 3000.0, 300.0, 0.05, False),
+                                                          (384, 128, 450.0, 150.0, 0.15, True),
+                                                          (128, 128, 2000.0, 25.0, 0.02, False)])]
+    warm, steps = 3, 17
+    mixed = [gpu.SwmGpu(s, precision).init_from_setup() for s in setups]
+    b = gpu.SwmBatch()
+    for m in mixed:
+        b.attach(m, warm + steps)
+    best, times = b.tune(steps=4)
+    assert best in (0, 1) and all(t > 0 for t in times)
+    assert all(m.itt == 0 for m in mixed)          # the tuning runs were undone
+    b.step(warm)
+    b.step(steps)
+    b.close()
+    for m, s in zip(mixed, setups):
+        a = gpu.SwmGpu(s, precision).init_from_setup()
+        a.step(warm + steps)
+        assert m.itt == a.itt == warm + steps
+        check_same(m, a, len(s.tracers), f"member {s.Nx}x{s.Ny}")
+        sm, sa = m.sums(0 if s.tracers else -1), a.sums(0 if s.tracers else -1)
+        assert np.isfinite(sm[0]) and np.isfinite(sm[1]), sm
+        assert same(np.array(sm), np.array(sa)), (sm, sa)
+        a.close()
+        m.close()
+
+
+def test_c4_ensemble_all_members(gpu_lib):
+    """The 64-member ensemble at its stated shapes (nx, ny from 128..1024, seed
+    0), attached to one batch: after 8 steps (the graph-replayed part
+    included) every member equals the same member stepped alone, bit for bit,
+    in FP64 and FP32; three members of different sizes also equal the
+    oracle."""
+    setups = configs.c4_ensemble(seed=0, n=64, nsteps=8)
+    assert len(setups) == 64 and len({(s.Nx, s.Ny) for s in setups}) > 10
+    for precision in (gpu.FP64, gpu.FP32):
+        mixed = [gpu.SwmGpu(s, precision).init_from_setup() for s in setups]
+        with gpu.SwmBatch() as b:
+            for m in mixed:
+                b.attach(m, 8)
+            assert b.step(3) == 64
+            assert b.step(10) == 0
+        for m, s in zip(mixed, setups):
+            a = gpu.SwmGpu(s, precision).init_from_setup()
+            a.step(8)
+            for f in ("SWM_U", "SWM_V", "SWM_ETA", "G_ETA"):
+                assert same(m.get(f), a.get(f)), (precision, s.Nx, s.Ny, f)
+            if s.tracers:
+                assert same(m.get("CH", 0), a.get("CH", 0)), (precision, s.Nx, s.Ny, "CH")
+            a.close()
+        if precision == gpu.FP64:
+            picks = sorted(range(64), key=lambda k: setups[k].Nx * setups[k].Ny)
+            for k in (picks[0], picks[32], picks[-1]):
+                o = oracle.Oracle(setups[k])
+                o.step(8)
+                assert same(mixed[k].get("SWM_ETA"), o.get("eta")) and same(mixed[k].get("SWM_U"), o.get("u"))
+                o.close()
+        for m in mixed:
+            m.close()

@@ -217,8 +217,11 @@ def test_sor_fixed_omega_fp64_and_fp32_sweeps(gpu_lib):
     chi_g, it_g, rr_g, ok = g.sor_fixed(B, np.zeros(s.shape), 1.9, 1e-10, 5000)
     assert ok and rc_o == 0
     assert abs(it_g - it_o) <= 1
-    if it_g == it_o:
-        assert same(chi_g, chi_o)
+    # chi bit for bit at a fixed sweep count: tolerance 0, both run exactly 300 sweeps
+    chi_o3, it_o3, _, _ = o.sor_fixed(B, np.zeros(s.shape), 1.9, 0.0, 300)
+    chi_g3, it_g3, _, _ = g.sor_fixed(B, np.zeros(s.shape), 1.9, 0.0, 300)
+    assert it_o3 == it_g3 == 300
+    assert same(chi_g3, chi_o3)
     chi_f, it_f, rr_f, ok_f = g.sor_fixed(B, np.zeros(s.shape), 1.9, 1e-6, 5000, fp32_sweeps=True)
     assert ok_f
     oc = s.grids["eta"].ocean == 1
@@ -264,9 +267,12 @@ def test_velocity_potential_device_resident(gpu_lib):
     it_g, rr_g, ok = g.velocity_potential(1.9, 1e-10, 5000, reset=True)
     assert ok and rc_o == 0
     assert abs(it_g - it_o) <= 1
-    if it_g == it_o:
-        assert same(g.get("CHI"), chi_o)
     chi64 = g.get("CHI")
+    # the device-resident solve bit for bit at a fixed sweep count (tolerance 0)
+    chi_o3, _, _, _ = o.sor_fixed(B, np.zeros(s.shape), 1.9, 0.0, 300)
+    it_g3, _, ok3 = g.velocity_potential(1.9, 0.0, 300, reset=True)
+    assert it_g3 == 300 and not ok3
+    assert same(g.get("CHI"), chi_o3)
     # FP32 sweeps: chi in FP32 bounds the attainable relative residual (about
     # 1e-6 here), so the run is judged by its residual and its distance to FP64
     it_f, rr_f, ok_f = g.velocity_potential(1.9, 1e-6, 5000, fp32_sweeps=True, reset=True)

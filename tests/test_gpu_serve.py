@@ -51,3 +51,31 @@ def test_served_results_equal_alone(gpu_lib, precision, budget):
     if budget < 1e9:
         # the budget holds about one request: admission waited for retirements
         assert any(r.t_admit > min(q.t_done for q in reqs) for r in reqs)
+
+
+def test_c5_stream_slice_with_largest_grid(gpu_lib):
+    """A slice of the serving stream's own request sequence (seed 0, 20 req/s,
+    realtime arrivals, FP32 fields) whose first request is given the stream's
+    largest grid, 2048 x 1024, and 3000 steps; the later requests keep their
+    grids with run lengths cut to 40..80 steps, so they arrive while the large
+    one is on the GPU.  Every result equals the same request run alone, and
+    the large request shares its launches with others."""
+    reqs = serve.request_stream(seed=0, rate=20.0, duration=0.5)[:6]
+    assert len(reqs) >= 4
+    reqs[0].Nx, reqs[0].Ny, reqs[0].nsteps = 2048, 1024, 3000
+    for k, r in enumerate(reqs[1:]):
+        r.nsteps = 40 + 10 * k
+    for r in reqs:
+        r.keep = True
+    srv = serve.Server(precision=gpu.FP32, hbm_budget=200e9, chunk=16)
+    stats = srv.run(reqs, realtime=True)
+    assert stats["requests"] == len(reqs)
+    assert stats["max_members_per_launch"] >= 2
+    big = reqs[0]
+    assert big.max_batch >= 2, "the 2048 x 1024 request never shared a launch"
+    for r in reqs:
+        assert r.result["itt"] == r.nsteps
+        ref = alone(r, gpu.FP32)
+        for k in r.result:
+            if k != "itt":
+                assert same(r.result[k], ref[k]), (r.rid, r.Nx, r.Ny, k)

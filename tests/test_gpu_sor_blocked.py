@@ -79,8 +79,34 @@ def test_blocked_converges_like_unblocked(gpu_lib, monkeypatch):
     chi_b, it_b, rr_b, ok_b = res["1"]
     chi_u, it_u, rr_u, ok_u = res["0"]
     assert ok_b and ok_u and it_b % 8 == 0 and abs(it_b - it_u) <= 8 and rr_b < 1e-9
-    if it_b == it_u:
-        assert same(chi_b, chi_u)
+
+
+@pytest.mark.parametrize("every,stop", [(2, 20), (2, 22), (4, 24), (4, 28), (2, 32), (4, 96)])
+def test_blocked_stops_at_chosen_test_sweep(gpu_lib, every, stop):
+    """Converging blocked runs with K = 2 and 4 sweeps per launch whose
+    tolerance is passed first at sweep `stop`: launch n does sweeps
+    nK+1..(n+1)K, so stop/K - 1 is odd for 20, 28, 32, 96 (the result is in
+    the caller's buffer) and even for 22, 24 (the result is copied back from
+    the second buffer); 32 and 96 end a host batch, whose test runs in the
+    one-block check instead of the next launch.  The tolerance is the geometric
+    mean of the oracle's relative residuals at sweeps stop - every and stop,
+    and every earlier test sweep's residual lies above it, so the run must stop
+    at `stop` with the oracle's chi of that many sweeps, bit for bit."""
+    s = basin(96, 64, 0.05, seed=2)
+    o = oracle.Oracle(s)
+    B = rhs(o, s, 6)
+    rr = {k: o.sor_fixed(B, np.zeros(s.shape), 1.9, 0.0, k)[2] for k in range(every, stop + 1, every)}
+    tol = float(np.sqrt(rr[stop] * rr[stop - every]))
+    assert rr[stop] < tol and all(rr[k] > tol for k in rr if k < stop)
+    chi_o = o.sor_fixed(B, np.zeros(s.shape), 1.9, 0.0, stop)[0]
+    o.close()
+    g = gpu.SwmGpu(s)
+    g.set_sor_check_every(every)
+    chi_g, it_g, rr_g, ok = g.sor_fixed(B, np.zeros(s.shape), 1.9, tol, 5000)
+    assert ok and it_g == stop
+    assert same(chi_g, chi_o)
+    assert abs(rr_g - rr[stop]) <= 1e-12 * rr[stop]
+    g.close()
 
 
 @pytest.mark.parametrize("blocked", ["1", "0"])
