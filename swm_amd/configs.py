@@ -33,16 +33,18 @@ def _smooth_noise(rng, Ny, Nx, sigma):
 
 def basin(Nx, Ny, dx, dy, *, variant, H0, dt, G=9.81, Ah=0.0, lbc=2.0, f0=0.0, beta=0.0,
           drag=0.0, lateral_mixing=False, land_fraction=0.0, seed=0, A=A_FLAT,
-          quadratic_friction=False, gamma_sq=0.0, psi_bs=None):
+          quadratic_friction=False, gamma_sq=0.0, psi_bs=None, side_walls=True):
     """A closed flat-bottom basin of depth H0 with optional seeded islands.
     LINEARISED_MEAN_STATE takes its basic state from the streamfunction psi_bs
-    (zero if None)."""
+    (zero if None).  side_walls=False leaves the zonal boundary open: a channel
+    periodic in x between the land rows that initDomain puts at j = 1, Ny."""
     rng = np.random.default_rng(seed)
     lon_e = _extent_deg(Nx, dx, A)
     lat_e = _extent_deg(Ny, dy, A)
     Hin = np.full((Ny, Nx), float(H0))
-    Hin[:, 0] = 0.0
-    Hin[:, -1] = 0.0
+    if side_walls:
+        Hin[:, 0] = 0.0
+        Hin[:, -1] = 0.0
     if land_fraction > 0.0:
         noise = _smooth_noise(rng, Ny, Nx, sigma=max(2.0, min(Nx, Ny) / 128.0))
         thr = np.quantile(noise, 1.0 - land_fraction)
