@@ -257,16 +257,115 @@ int fail(swm_ctx* c, const std::string& msg, int code = 2) {
             return fail((ctx), std::string(#call) + ": " + hipGetErrorString(e_), 3);          \
     } while (0)
 
-// Every copy and fill runs on the context's own stream and is waited for.
-// The stream is non-blocking, so work on the legacy null stream (plain
-// hipMemcpy / hipMemset) would not be ordered with the kernels on it.
+// Host transfers.  Every copy between host and device memory passes through
+// one pinned (page-locked) staging buffer of the library and is a plain 1-D
+// DMA copy on the stream that orders it with the kernels (the contexts' and
+// batches' streams are non-blocking, so legacy null-stream copies would not
+// be ordered with them), waited for before the call returns.  Caller memory
+// is only ever touched by host memcpy: the DMA engines never see pageable
+// memory, and pitched planes are packed and unpacked on the host.
+constexpr size_t kStageMax = (size_t)32 << 20;
+char* g_stage = nullptr;
+size_t g_stage_bytes = 0;
+
+// A staging buffer of at least min(want, kStageMax) bytes (callers hold
+// g_hip_mu).  grow = false never allocates: it fails when the buffer (made by
+// the first upload of any context, at least 1 MiB) is missing or smaller.
+hipError_t stage_get(size_t want, char** p, size_t* n, bool grow = true) {
+    want = std::max<size_t>(std::min(want, kStageMax), (size_t)1 << 20);
+    if (g_stage_bytes < want && !grow) return hipErrorOutOfMemory;
+    if (g_stage_bytes < want) {
+        if (g_stage) (void)hipHostFree(g_stage);
+        g_stage = nullptr;
+        g_stage_bytes = 0;
+        hipError_t e = hipHostMalloc((void**)&g_stage, want, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        g_stage_bytes = want;
+    }
+    *p = g_stage;
+    *n = g_stage_bytes;
+    return hipSuccess;
+}
+
+hipError_t h2d(hipStream_t s, void* dst, const void* src, size_t bytes, bool grow = true) {
+    HipLock lock(g_hip_mu);
+    char* st = nullptr;
+    size_t cap = 0;
+    hipError_t e = stage_get(bytes, &st, &cap, grow);
+    for (size_t off = 0; e == hipSuccess && off < bytes; off += cap) {
+        const size_t len = std::min(cap, bytes - off);
+        std::memcpy(st, (const char*)src + off, len);
+        e = hipMemcpyAsync((char*)dst + off, st, len, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    return e;
+}
+
+hipError_t d2h(hipStream_t s, void* dst, const void* src, size_t bytes, bool grow = true) {
+    HipLock lock(g_hip_mu);
+    char* st = nullptr;
+    size_t cap = 0;
+    hipError_t e = stage_get(bytes, &st, &cap, grow);
+    for (size_t off = 0; e == hipSuccess && off < bytes; off += cap) {
+        const size_t len = std::min(cap, bytes - off);
+        e = hipMemcpyAsync(st, (const char*)src + off, len, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e == hipSuccess) std::memcpy((char*)dst + off, st, len);
+    }
+    return e;
+}
+
+// Rows of `width` bytes: host rows spaced spitch, device rows spaced dpitch
+// (dpitch >= width; the padding bytes of each device row are written as zero).
+hipError_t h2d_rows(hipStream_t s, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                    size_t rows) {
+    HipLock lock(g_hip_mu);
+    char* st = nullptr;
+    size_t cap = 0;
+    hipError_t e = stage_get(rows * dpitch, &st, &cap);
+    const size_t per = std::max<size_t>(1, cap / dpitch);
+    if (e == hipSuccess && per * dpitch > cap) return hipErrorInvalidValue;   // one row must fit
+    for (size_t r0 = 0; e == hipSuccess && r0 < rows; r0 += per) {
+        const size_t nr = std::min(per, rows - r0);
+        for (size_t r = 0; r < nr; ++r) {
+            std::memcpy(st + r * dpitch, (const char*)src + (r0 + r) * spitch, width);
+            if (dpitch > width) std::memset(st + r * dpitch + width, 0, dpitch - width);
+        }
+        e = hipMemcpyAsync((char*)dst + r0 * dpitch, st, nr * dpitch, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    return e;
+}
+
+hipError_t d2h_rows(hipStream_t s, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                    size_t rows) {
+    HipLock lock(g_hip_mu);
+    char* st = nullptr;
+    size_t cap = 0;
+    hipError_t e = stage_get(rows * spitch, &st, &cap);
+    const size_t per = std::max<size_t>(1, cap / spitch);
+    if (e == hipSuccess && per * spitch > cap) return hipErrorInvalidValue;
+    for (size_t r0 = 0; e == hipSuccess && r0 < rows; r0 += per) {
+        const size_t nr = std::min(per, rows - r0);
+        e = hipMemcpyAsync(st, (const char*)src + r0 * spitch, nr * spitch, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e == hipSuccess)
+            for (size_t r = 0; r < nr; ++r) std::memcpy((char*)dst + (r0 + r) * dpitch, st + r * spitch, width);
+    }
+    return e;
+}
+
 hipError_t copy_sync(swm_ctx* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    if (kind == hipMemcpyHostToDevice) return h2d(c->stream, dst, src, bytes);
+    if (kind == hipMemcpyDeviceToHost) return d2h(c->stream, dst, src, bytes);
     hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
     return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
 }
 
 hipError_t copy2d_sync(swm_ctx* c, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
                        size_t rows, hipMemcpyKind kind) {
+    if (kind == hipMemcpyHostToDevice) return h2d_rows(c->stream, dst, dpitch, src, spitch, width, rows);
+    if (kind == hipMemcpyDeviceToHost) return d2h_rows(c->stream, dst, dpitch, src, spitch, width, rows);
     hipError_t e = hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, kind, c->stream);
     return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
 }
@@ -1199,9 +1298,8 @@ int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_o
         if (tracer_id >= 0) return fail(c, "PSI is not a tracer field", 1);
         int rc = c->fp32 ? psi_t<float>(c) : psi_t<double>(c);
         if (rc) return rc;
-        HIPCHK(c, hipMemcpy2DAsync(host_out, c->Nx * 8, c->psi_buf[3], c->P * 8, c->Nx * 8, c->Ny,
-                                   hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, copy2d_sync(c, host_out, c->Nx * 8, c->psi_buf[3], c->P * 8, c->Nx * 8, c->Ny,
+                              hipMemcpyDeviceToHost));
         return 0;
     }
     if (s == "CHI") {   // the last device-resident SOR solution
@@ -1406,8 +1504,7 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
                                    (const double*)c->sor_partial + (size_t)((n - 1) & 1) * nb, nb, (int)pending, tol,
                                    bnorm, ref_mode ? 0 : 1, c->sor_done, c->sor_iters, c->sor_norm);
             HIPCHK(c, hipGetLastError());
-            HIPCHK(c, hipMemcpyAsync(host_done, c->sor_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, copy_sync(c, host_done, c->sor_done, sizeof(int), hipMemcpyDeviceToHost));
         }
         if (*host_done) {   // the buffer of the launch whose test passed
             long long it = 0;
@@ -1444,8 +1541,7 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
                                    c->sor_norm);
         }
         HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipMemcpyAsync(host_done, c->sor_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, copy_sync(c, host_done, c->sor_done, sizeof(int), hipMemcpyDeviceToHost));
     }
     return 0;
 }
@@ -1537,10 +1633,9 @@ int velpot_t(swm_ctx* c, double omega, double rel_tol, int64_t max_sweeps, bool 
                        c->norm_buf + 1024);
     HIPCHK(c, hipGetLastError());
     double bnorm = 0.0;
-    HIPCHK(c, hipMemcpyAsync(&bnorm, c->norm_buf + 1024, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemsetAsync(c->sor_done, 0, sizeof(int), c->stream));
     HIPCHK(c, hipMemsetAsync(c->sor_iters, 0, sizeof(long long), c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, copy_sync(c, &bnorm, c->norm_buf + 1024, sizeof(double), hipMemcpyDeviceToHost));
     int host_done = 0;
     if ((rc = sor_sweeps<TS>(c, dchi, dB, c->sor_ftab[w], false, omega, rel_tol, bnorm, max_sweeps, nblk, &host_done)))
         return rc;
@@ -1553,6 +1648,34 @@ int velpot_t(swm_ctx* c, double omega, double rel_tol, int64_t max_sweeps, bool 
     c->chi_valid[w] = true;
     c->chi_last = w;
     return host_done ? 0 : 1;
+}
+
+// FP64 conservation sums of the current state into dout (3 values, copied to
+// the host out): sums_kernel's per-block partials in `partial` (6 per block,
+// at most 1024 blocks), the eta rows' cos(lat) in `ce`.
+int sums_launch(swm_ctx* c, int32_t tracer_id, double* partial, double* dout, const double* ce, double* out,
+                bool grow = true) {
+    const long n = (long)c->Nx * c->Ny;
+    const int nb = (int)std::min<long>(1024, (n + 255) / 256);
+    const int fn = c->d.variant == SWM_FULLY_NONLINEAR;
+    const int ch_plane = tracer_id >= 0 ? swm::PL_TRC + swm::PL_PER_TRC * tracer_id + c->cur : -1;
+    if (c->fp32)
+        hipLaunchKernelGGL(swm::sums_kernel<float>, dim3(nb), dim3(256), 0, c->stream,
+                           (const Statics<float>*)c->statics, fn, (const float*)plane_ptr(c, swm::PL_E + c->cur),
+                           (const float*)plane_ptr(c, swm::PL_U + c->cur), (const float*)plane_ptr(c, swm::PL_V + c->cur),
+                           ch_plane >= 0 ? (const float*)plane_ptr(c, ch_plane) : nullptr, ce, partial);
+    else
+        hipLaunchKernelGGL(swm::sums_kernel<double>, dim3(nb), dim3(256), 0, c->stream,
+                           (const Statics<double>*)c->statics, fn, (const double*)plane_ptr(c, swm::PL_E + c->cur),
+                           (const double*)plane_ptr(c, swm::PL_U + c->cur),
+                           (const double*)plane_ptr(c, swm::PL_V + c->cur),
+                           ch_plane >= 0 ? (const double*)plane_ptr(c, ch_plane) : nullptr, ce, partial);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(swm::sums_final, dim3(1), dim3(256), 0, c->stream, (const double*)partial, nb, dout);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, d2h(c->stream, out, dout, 3 * sizeof(double), grow));
+    if (tracer_id < 0) out[2] = 0.0;
+    return 0;
 }
 
 }  // namespace
@@ -1776,38 +1899,13 @@ int swm_gpu_sums(swm_ctx* c, int32_t tracer_id, double* out) {
     if (!c || !out) return 1;
     if (int rc = member_refresh(c)) return rc;
     if (tracer_id >= (int32_t)c->trc.size()) return fail(c, "tracer id out of range", 1);
-    const long n = (long)c->Nx * c->Ny;
-    const int nb = (int)std::min<long>(1024, (n + 255) / 256);
     if (!c->sums_buf) {
         c->sums_buf = (double*)dalloc(c, sizeof(double) * (6 * 1024 + 3));
         c->ce64 = (double*)dalloc(c, sizeof(double) * c->Ny);
         if (!c->sums_buf || !c->ce64) return fail(c, "out of device memory", 3);
         HIPCHK(c, copy_sync(c, c->ce64, c->ce_host.data(), sizeof(double) * c->Ny, hipMemcpyHostToDevice));
     }
-    const int fn = c->d.variant == SWM_FULLY_NONLINEAR;
-    const int ch_plane = tracer_id >= 0 ? swm::PL_TRC + swm::PL_PER_TRC * tracer_id + c->cur : -1;
-    double* partial = c->sums_buf;
-    double* dout = c->sums_buf + 6 * 1024;
-    if (c->fp32)
-        hipLaunchKernelGGL(swm::sums_kernel<float>, dim3(nb), dim3(256), 0, c->stream,
-                           (const Statics<float>*)c->statics, fn, (const float*)plane_ptr(c, swm::PL_E + c->cur),
-                           (const float*)plane_ptr(c, swm::PL_U + c->cur), (const float*)plane_ptr(c, swm::PL_V + c->cur),
-                           ch_plane >= 0 ? (const float*)plane_ptr(c, ch_plane) : nullptr, (const double*)c->ce64,
-                           partial);
-    else
-        hipLaunchKernelGGL(swm::sums_kernel<double>, dim3(nb), dim3(256), 0, c->stream,
-                           (const Statics<double>*)c->statics, fn, (const double*)plane_ptr(c, swm::PL_E + c->cur),
-                           (const double*)plane_ptr(c, swm::PL_U + c->cur),
-                           (const double*)plane_ptr(c, swm::PL_V + c->cur),
-                           ch_plane >= 0 ? (const double*)plane_ptr(c, ch_plane) : nullptr, (const double*)c->ce64,
-                           partial);
-    HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(swm::sums_final, dim3(1), dim3(256), 0, c->stream, (const double*)partial, nb, dout);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(out, dout, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (tracer_id < 0) out[2] = 0.0;
-    return 0;
+    return sums_launch(c, tracer_id, c->sums_buf, c->sums_buf + 6 * 1024, c->ce64, out);
 }
 
 const char* swm_gpu_last_error(const swm_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
@@ -1881,11 +1979,8 @@ void batch_drop_graph(swm_batch* b) {
 int batch_pull(swm_batch* b) {
     if (!b->on_device) return 0;
     BHIPCHK(b, hipStreamSynchronize(b->stream));
-    if (!b->hstep.empty()) {
-        BHIPCHK(b, hipMemcpyAsync(b->hstep.data(), b->dstep, b->hstep.size() * sizeof(swm::MemberStep),
-                                  hipMemcpyDeviceToHost, b->stream));
-        BHIPCHK(b, hipStreamSynchronize(b->stream));
-    }
+    if (!b->hstep.empty())
+        BHIPCHK(b, d2h(b->stream, b->hstep.data(), b->dstep, b->hstep.size() * sizeof(swm::MemberStep)));
     b->on_device = false;
     for (size_t k = 0; k < b->mem.size(); ++k) {
         swm_ctx* c = b->mem[k];
@@ -1944,13 +2039,10 @@ int batch_build_t(swm_batch* b) {
     std::vector<int> owner((size_t)work);
     for (size_t k = 0; k < n; ++k)
         std::fill(owner.begin() + hm[k].wbase, owner.begin() + hm[k].wbase + hm[k].ntile, (int)k);
-    if (work)
-        BHIPCHK(b, hipMemcpyAsync(b->downer, owner.data(), (size_t)work * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    if (work) BHIPCHK(b, h2d(b->stream, b->downer, owner.data(), (size_t)work * sizeof(int)));
     if (n) {
-        BHIPCHK(b, hipMemcpyAsync(b->dmem, hm.data(), n * sizeof(swm::Member<T>), hipMemcpyHostToDevice, b->stream));
-        BHIPCHK(b, hipMemcpyAsync(b->dstep, b->hstep.data(), n * sizeof(swm::MemberStep), hipMemcpyHostToDevice,
-                                  b->stream));
-        BHIPCHK(b, hipStreamSynchronize(b->stream));
+        BHIPCHK(b, h2d(b->stream, b->dmem, hm.data(), n * sizeof(swm::Member<T>)));
+        BHIPCHK(b, h2d(b->stream, b->dstep, b->hstep.data(), n * sizeof(swm::MemberStep)));
     }
     b->on_device = n > 0;
     b->dirty = false;
