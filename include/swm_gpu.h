@@ -231,6 +231,16 @@ int swm_gpu_divergence(swm_ctx *c, double *host_div);
  * A^2*dLambda*dTheta for volumes.  Blocks until the result is on the host. */
 int swm_gpu_sums(swm_ctx *c, int32_t tracer_id, double *out);
 
+/* The same sums as swm_gpu_sums without any device allocation: no
+ * counterpart in the reference; it serves callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Its partial sums, result and cos(lat) rows live in
+ * a static device array of the library (one per GPU, about 180 KB, held
+ * until the process ends), so grids of at most 16384 rows (Ny) are accepted;
+ * larger ones fail with status 4.  Calls that overlap on one GPU (from
+ * several host threads or contexts at once) share that array and race. */
+int swm_gpu_sums_noalloc(swm_ctx *c, int32_t tracer_id, double *out);
+
 /* Batched ensembles.  No counterpart in the reference, which advances one
  * domain per process (model.f90:159-196 per member); these serve callers that
  * run many independent domains of uneven size at once (ensembles, a request

@@ -1678,6 +1678,38 @@ int sums_launch(swm_ctx* c, int32_t tracer_id, double* partial, double* dout, co
     return 0;
 }
 
+// Scratch of swm_gpu_sums_noalloc: a file-scope device array, one copy per
+// GPU, held until the process ends.  Layout (doubles): [0, 6144) per-block
+// partials, [6144, 6147) the result, [6152, 6152 + kNoAllocMaxRows) the eta
+// rows' cos(lat).  Every call on a GPU uses that GPU's copy, so calls that
+// overlap on one GPU (two host threads, two streams) race.
+constexpr int kNoAllocMaxRows = 16384;
+constexpr size_t kScratchResult = 6 * 1024;
+constexpr size_t kScratchRows = 6 * 1024 + 8;
+constexpr size_t kScratchDoubles = kScratchRows + kNoAllocMaxRows;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the host-side name is only a stand-in for the symbol).
+int scratch_ptr(swm_ctx* c, double** out) {
+    int dev = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(c, "device index beyond the scratch table", 4);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIPCHK(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(c, "hipGetSymbolAddress(scratch) returned no address", 3);
+        g_scratch_dev[dev] = (double*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1906,6 +1938,27 @@ int swm_gpu_sums(swm_ctx* c, int32_t tracer_id, double* out) {
         HIPCHK(c, copy_sync(c, c->ce64, c->ce_host.data(), sizeof(double) * c->Ny, hipMemcpyHostToDevice));
     }
     return sums_launch(c, tracer_id, c->sums_buf, c->sums_buf + 6 * 1024, c->ce64, out);
+}
+
+int swm_gpu_sums_noalloc(swm_ctx* c, int32_t tracer_id, double* out) {
+    if (!c || !out) return 1;
+    if (c->Ny > kNoAllocMaxRows)
+        return fail(c, "swm_gpu_sums_noalloc: Ny = " + std::to_string(c->Ny) + " exceeds the " +
+                       std::to_string(kNoAllocMaxRows) + " rows of its static scratch", 4);
+    HipLock lock(g_hip_mu);
+    if (int rc = member_refresh(c)) return rc;
+    if (tracer_id >= (int32_t)c->trc.size()) return fail(c, "tracer id out of range", 1);
+    double* ws = nullptr;
+    if (int rc = scratch_ptr(c, &ws)) return rc;
+    double* partial = ws;
+    double* dout = ws + kScratchResult;
+    double* ce = ws + kScratchRows;
+    // the cos rows are this context's; the partials and the result are
+    // rewritten in full by the kernels, so nothing needs clearing.  The host
+    // transfers use the staging buffer that the context's creation made (at
+    // least 1 MiB, more than these 16384 * 8 + 24 bytes) and never grow it.
+    HIPCHK(c, h2d(c->stream, ce, c->ce_host.data(), sizeof(double) * c->Ny, false));
+    return sums_launch(c, tracer_id, partial, dout, ce, out, false);
 }
 
 const char* swm_gpu_last_error(const swm_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }

@@ -36,7 +36,7 @@ EXPORTED = [
     "swm_gpu_batch_remaining", "swm_gpu_batch_step", "swm_gpu_batch_last_timing", "swm_gpu_batch_last_error",
     "swm_gpu_batch_destroy", "swm_gpu_kernel_shape_count", "swm_gpu_set_kernel_shape", "swm_gpu_tune",
     "swm_gpu_set_sor_check_every", "swm_gpu_batch_set_shape", "swm_gpu_batch_tune", "swm_gpu_avg_add",
-    "swm_gpu_avg_read", "swm_gpu_add_forcing_stream",
+    "swm_gpu_avg_read", "swm_gpu_add_forcing_stream", "swm_gpu_sums_noalloc",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -110,6 +110,8 @@ def load_library(path: str = LIB_PATH):
     L.swm_gpu_divergence.restype = C.c_int
     L.swm_gpu_sums.argtypes = [p, C.c_int32, _dp]
     L.swm_gpu_sums.restype = C.c_int
+    L.swm_gpu_sums_noalloc.argtypes = [p, C.c_int32, _dp]
+    L.swm_gpu_sums_noalloc.restype = C.c_int
     L.swm_gpu_last_timing.argtypes = [p, _dp, _dp]
     L.swm_gpu_last_timing.restype = C.c_int
     L.swm_gpu_last_error.argtypes = [p]
