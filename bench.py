@@ -280,6 +280,22 @@ def bench_elliptic(args):
                 rec[tag + "_chi"] = chi - chi.mean()
     ref = rec["f64_chi"]
     err = float(np.abs(rec["f32_chi"] - ref).max() / max(np.abs(ref).max(), 1e-300))
+    # the reference's own solve, main_ElSolv_SOR as computeVelocityPotential
+    # calls it (calc_lib.f90:500-525): Chebyshev omega, stop when
+    # sqrt(sum res^2) < EPS = 1e-4, first guess only at itt = 1, chi kept
+    # between steps; iterations to that tolerance per step, on the same
+    # trajectory continued for the same number of steps
+    chi_ref = np.zeros(s.shape)
+    ref_it, ref_t = [], 0.0
+    for k in range(args.steps):
+        g.step(1)
+        B = g.divergence()
+        t0 = time.perf_counter()
+        chi_ref, it, an, ok = g.sor(B, chi_ref, epsilon=1e-4, first_guess=(k == 0))
+        ref_t += time.perf_counter() - t0
+        ref_it.append((it, an, ok))
+        print(f"elliptic reference-mode step {k + 1}/{args.steps}: {it} sweeps, anorm {an:.3g}", file=sys.stderr,
+              flush=True)
     cells = s.Nx * s.Ny
     sweeps = {t: sum(r[0] for r in rec[t]) for t in ("f64", "f32")}
     per_sweep = {t: t_sor[t] / max(sweeps[t], 1) for t in sweeps}
@@ -299,6 +315,9 @@ def bench_elliptic(args):
         "fp32_sweeps_chi_rel_error_vs_fp64": err,
         "autotune_sor": tuned,
         "model_step_ms": t_step / args.steps * 1e3,
+        "reference_mode": {"sweeps_per_step": [r[0] for r in ref_it], "anorm": [r[1] for r in ref_it],
+                           "converged": [bool(r[2]) for r in ref_it], "epsilon": 1e-4,
+                           "sweep_us_incl_host_copies": ref_t / max(sum(r[0] for r in ref_it), 1) * 1e6},
     }
     print(json.dumps(out))
     g.close()

@@ -41,7 +41,8 @@ enum swm_flags {
     SWM_LATERAL_MIXING = 1,                 /* LATERAL_MIXING */
     SWM_QUADRATIC_BOTTOM_FRICTION = 2,      /* QUADRATIC_BOTTOM_FRICTION */
     SWM_BAROTROPIC = 4,                     /* BAROTROPIC */
-    SWM_TAU_SCALE = 8                       /* TAU_SCALE defined (value in swm_desc.tau_scale) */
+    SWM_TAU_SCALE = 8,                      /* TAU_SCALE defined (value in swm_desc.tau_scale) */
+    SWM_ELLIPTIC_SOLVER = 16                /* built with CALC_LIB_ELLIPTIC_SOLVER = ElSolv_SOR (include/calc_lib.h:7-18) */
 };
 enum swm_precision { SWM_FP64 = 0, SWM_FP32 = 1 };
 enum swm_grid_id { SWM_GRID_H = 0, SWM_GRID_U = 1, SWM_GRID_V = 2, SWM_GRID_ETA = 3 };
@@ -150,10 +151,18 @@ int64_t swm_gpu_itt(const swm_ctx *c);
  * SWM_D SWM_DH SWM_DU SWM_DV SWM_EDENS SWM_POT SWM_RELVORT SWM_MU SWM_MV
  * F_X F_Y F_ETA SWM_LATMIX_U SWM_LATMIX_V, and for tracer_id >= 0:
  * CH C G_CH uhc vhc.  Derived fields are materialised on demand from the
- * device state.  PSI is the diagnostic streamfunction of u, v at N0
- * (diagVar DVARNAME_PSI -> computeStreamfunction, src/diagVar.f90:122-123,
- * src/calc_lib.f90:727-780, as built without an elliptic solver), computed on
- * the device in FP64.  Blocks. */
+ * device state.  The diagnostic variables of src/diagVar.f90:115-134:
+ *   PSI   computeStreamfunction (src/calc_lib.f90:727-780) of u, v at N0,
+ *         on the device in FP64; with SWM_ELLIPTIC_SOLVER of the non-divergent
+ *         flow u_nd, v_nd (as that build's computeStreamfunction does);
+ *   U_ND, V_ND  computeNonDivergentFlowField (src/calc_lib.f90:548-590):
+ *         u - d(chi)/dx, v - d(chi)/dy with SWM_ELLIPTIC_SOLVER, else u, v;
+ *   CHI   with SWM_ELLIPTIC_SOLVER computeVelocityPotential
+ *         (src/calc_lib.f90:500-525): main_ElSolv_SOR of the divergence of u,
+ *         v with epsilon = EPS and first_guess = (itt == 1), chi kept on the
+ *         device between steps, solved at most once per step; without it, the
+ *         last swm_gpu_velocity_potential solution.
+ * Blocks. */
 int swm_gpu_sync(swm_ctx *c, const char *name, int32_t tracer_id, double *host_out);
 
 /* main_ElSolv_SOR(B, chi, epsilon, first_guess) contract
@@ -213,6 +222,11 @@ int swm_gpu_set_sor_check_every(swm_ctx *c, int32_t every);
  * Same return convention as swm_gpu_sor_fixed. */
 int swm_gpu_velocity_potential(swm_ctx *c, double omega, double rel_tol, int64_t max_sweeps, int32_t fp32_sweeps,
                                int32_t reset, int64_t *iters, double *rel_res);
+
+/* Sweeps used and final sqrt(sum res^2) of the last computeVelocityPotential
+ * solve of the elliptic-solver build (the reference logs them at debug level,
+ * src/ElSolv_SOR.f90:213-220). */
+int swm_gpu_elliptic_stats(swm_ctx *c, int64_t *iters, double *anorm);
 
 /* computeDivergence(u, v, div, u_grid, v_grid) of the current device velocity
  * (src/calc_lib.f90:607-622), the right-hand side computeVelocityPotential

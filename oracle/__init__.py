@@ -83,6 +83,7 @@ def lib():
         L.orc_sor_fixed.restype = C.c_int
         L.orc_divergence.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_psi.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.orc_nondivergent.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.orc_sor_apply.argtypes = [C.c_void_p, _dp, _dp]
         _lib = L
     return _lib
@@ -184,6 +185,17 @@ class Oracle:
         out = np.empty(self.s.shape)
         lib().orc_divergence(self._h, u.ctypes.data_as(_dp), v.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
         return out
+
+    def nondivergent(self, u, v, chi):
+        """computeNonDivergentFlowField of (u, v) given the velocity potential
+        chi (the elliptic-solver build): (u - dchi/dx, v - dchi/dy)."""
+        u = np.ascontiguousarray(u, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        chi = np.ascontiguousarray(chi, np.float64)
+        un, vn = np.empty(self.s.shape), np.empty(self.s.shape)
+        lib().orc_nondivergent(self._h, u.ctypes.data_as(_dp), v.ctypes.data_as(_dp), chi.ctypes.data_as(_dp),
+                               un.ctypes.data_as(_dp), vn.ctypes.data_as(_dp))
+        return un, vn
 
     def psi(self, u=None, v=None):
         """computeStreamfunction of (u, v); default the master u, v at N0."""

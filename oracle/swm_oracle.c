@@ -243,6 +243,23 @@ void orc_divergence(const orc_model *m, const double *u, const double *v, double
     free(a); free(cv); free(b);
 }
 
+/* computeNonDivergentFlowField with the elliptic solver (calc_lib.f90:548-590):
+ * u_nd = u - u_corr, v_nd = v - v_corr with (u_corr, v_corr) =
+ * computeGradient(chi, eta_grid) (calc_lib.f90:638-648), i.e. pder_zonal and
+ * pder_meridional of chi on the eta grid, given the velocity potential chi. */
+void orc_nondivergent(const orc_model *m, const double *u, const double *v, const double *chi, double *u_nd,
+                      double *v_nd) {
+    size_t n = (size_t)m->Nx * m->Ny;
+    double *uc = zalloc(n), *vc = zalloc(n);
+    pder_zonal(m, chi, ORC_GRID_ETA, uc);
+    pder_meridional(m, chi, ORC_GRID_ETA, vc);
+    for (size_t k = 0; k < n; ++k) {
+        u_nd[k] = u[k] - uc[k];
+        v_nd[k] = v[k] - vc[k];
+    }
+    free(uc); free(vc);
+}
+
 /* ---------------- SWM_timestep_module diagnostics ---------------- */
 
 /* computeD, swm_timestep_module.f90:182-218 */

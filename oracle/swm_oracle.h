@@ -91,4 +91,6 @@ void orc_sor_apply(orc_model *m, const double *chi, double *out);
 void orc_psi(const orc_model *m, const double *u, const double *v, double *psi);
 /* computeDivergence(u, v, div, u_grid, v_grid) */
 void orc_divergence(const orc_model *m, const double *u, const double *v, double *div);
+void orc_nondivergent(const orc_model *m, const double *u, const double *v, const double *chi, double *u_nd,
+                      double *v_nd);
 #endif

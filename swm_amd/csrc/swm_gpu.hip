@@ -202,7 +202,12 @@ struct swm_ctx {
     std::vector<Tracer> trc;
     std::vector<FStream> streams;   // time-interpolated forcing streams, in processing order
     int tile = 0;
-    bool use_graph = true;
+    // Steady steps replayed from captured six-step graphs (SWM_GPU_GRAPH=1).
+    // Off by default: on MI355X direct launches measured faster at every grid
+    // size from 128^2 to 1024^2 (e.g. 1024^2 FP64: 46.34 against 46.95 us per
+    // step; 128^2: 9.38 against 9.89 us), the graphs' per-launch overhead
+    // exceeding what they save in host work.
+    bool use_graph = false;
     int num_cu = 0;              // compute units of the device
     double* sums_buf = nullptr;  // conservation-sum partials and result
     double* norm_buf = nullptr;  // ||B|| partials and result of the device-resident SOR
@@ -240,6 +245,17 @@ struct swm_ctx {
     void* sor_alt[2] = {};       // second chi buffer of the blocked sweeps (ping-pong)
     void* sor_ftab[2][5] = {};   // coefficient tables in the sweep precision
     void* dza = nullptr;
+    // elliptic-solver build (SWM_ELLIPTIC_SOLVER): the reference-mode velocity
+    // potential kept between steps (calc_lib::chi), solved at most once per
+    // step when a diagnostic asks for it, and the non-divergent flow
+    void* chi_ref = nullptr;     // chi, context precision
+    void* chi_refB = nullptr;    // its right-hand side
+    double* nd_buf[2] = {};      // u_nd, v_nd (FP64)
+    double* gza = nullptr;       // bc / (A cos_u dLambda) per row, ocean and land
+    std::vector<double> cu_host; // cos(lat_u) rows
+    int64_t chi_ref_itt = -1;    // itt of the last solve (-1: none)
+    int64_t chi_ref_iters = 0;
+    double chi_ref_anorm = 0.0;
     swm_batch* batch = nullptr;  // batch this context is attached to
 };
 
@@ -1136,6 +1152,7 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
     if ((rc = alloc_state(c, 0))) return rc;
     if (!(c->scratch = alloc_grid(c))) return fail(c, "out of device memory", 3);
     c->ce_host.assign(d->grid[SWM_GRID_ETA].cos_lat, d->grid[SWM_GRID_ETA].cos_lat + c->Ny);
+    c->cu_host.assign(d->grid[SWM_GRID_U].cos_lat, d->grid[SWM_GRID_U].cos_lat + c->Ny);
     // host pointers in the descriptor are only valid during this call
     for (int g = 0; g < 4; ++g) c->d.grid[g] = swm_grid_desc{};
     c->d.impl_u = c->d.impl_v = c->d.impl_eta = c->d.gamma_sq_u = c->d.gamma_sq_v = nullptr;
@@ -1260,9 +1277,13 @@ int swm_gpu_last_timing(const swm_ctx* c, double* total_ms, double* step_kernel_
 
 namespace {
 
-// computeStreamfunction of the master u, v at N0 into psi_buf[3].
+int elliptic_chi(swm_ctx* c);
+int elliptic_nd(swm_ctx* c);
+
+// computeStreamfunction of the master u, v at N0 into psi_buf[3]; und / vnd
+// (FP64 planes, optional) replace u, v by the non-divergent flow.
 template <typename T>
-int psi_t(swm_ctx* c) {
+int psi_t(swm_ctx* c, const double* und = nullptr, const double* vnd = nullptr) {
     const size_t bytes = (size_t)c->plane * sizeof(double);
     for (auto& b : c->psi_buf)
         if (!b && !(b = (double*)dalloc(c, bytes))) return fail(c, "out of device memory", 3);
@@ -1270,8 +1291,8 @@ int psi_t(swm_ctx* c) {
     const T* u = (const T*)plane_ptr(c, swm::PL_U + c->cur);
     const T* v = (const T*)plane_ptr(c, swm::PL_V + c->cur);
     const long n = (long)c->Nx * c->Ny;
-    hipLaunchKernelGGL(swm::psi_prep<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, sp, u, v,
-                       c->psi_buf[0], c->psi_buf[1]);
+    hipLaunchKernelGGL(swm::psi_prep<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, sp, u, v, und,
+                       vnd, c->psi_buf[0], c->psi_buf[1]);
     hipLaunchKernelGGL(swm::psi_col<T>, dim3((c->Nx + 255) / 256), dim3(256), 0, c->stream, sp,
                        (const double*)c->psi_buf[0], c->psi_buf[2], c->psi_buf[3]);
     const unsigned segs = (unsigned)((c->Nx - 1 + 255) / 256);
@@ -1294,12 +1315,38 @@ int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_o
     if (int rc = member_refresh(c)) return rc;
     std::string s(name);
     for (auto& ch : s) ch = (char)toupper((unsigned char)ch);
+    const bool ell = (c->d.flags & SWM_ELLIPTIC_SOLVER) != 0;
     if (s == "PSI") {   // computeStreamfunction of u, v at N0 (diagVar DVARNAME_PSI)
         if (tracer_id >= 0) return fail(c, "PSI is not a tracer field", 1);
-        int rc = c->fp32 ? psi_t<float>(c) : psi_t<double>(c);
+        int rc = 0;
+        if (ell) {   // of the non-divergent flow u - grad chi
+            if ((rc = elliptic_nd(c))) return rc;
+            rc = c->fp32 ? psi_t<float>(c, c->nd_buf[0], c->nd_buf[1]) : psi_t<double>(c, c->nd_buf[0], c->nd_buf[1]);
+        } else {
+            rc = c->fp32 ? psi_t<float>(c) : psi_t<double>(c);
+        }
         if (rc) return rc;
         HIPCHK(c, copy2d_sync(c, host_out, c->Nx * 8, c->psi_buf[3], c->P * 8, c->Nx * 8, c->Ny,
                               hipMemcpyDeviceToHost));
+        return 0;
+    }
+    if ((s == "U_ND" || s == "V_ND") && ell) {   // computeNonDivergentFlowField (diagVar U_ND, V_ND)
+        if (tracer_id >= 0) return fail(c, s + " is not a tracer field", 1);
+        if (int rc = elliptic_nd(c)) return rc;
+        HIPCHK(c, copy2d_sync(c, host_out, c->Nx * 8, c->nd_buf[s == "U_ND" ? 0 : 1], c->P * 8, c->Nx * 8, c->Ny,
+                              hipMemcpyDeviceToHost));
+        return 0;
+    }
+    if (s == "CHI" && ell) {   // computeVelocityPotential (diagVar CHI): the reference-mode solve
+        if (int rc = elliptic_chi(c)) return rc;
+        const size_t n = (size_t)c->Nx * c->Ny;
+        if (c->fp32) {
+            std::vector<float> f(n);
+            HIPCHK(c, copy2d_sync(c, f.data(), c->Nx * 4, c->chi_ref, c->P * 4, c->Nx * 4, c->Ny, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < n; ++k) host_out[k] = f[k];
+        } else {
+            HIPCHK(c, copy2d_sync(c, host_out, c->Nx * 8, c->chi_ref, c->P * 8, c->Nx * 8, c->Ny, hipMemcpyDeviceToHost));
+        }
         return 0;
     }
     if (s == "CHI") {   // the last device-resident SOR solution
@@ -1332,8 +1379,8 @@ int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_o
         } else {
             return fail(c, "unknown tracer field " + s, 1);
         }
-    } else if (s == "SWM_U" || s == "U") src = plane_ptr(c, swm::PL_U + n0);
-    else if (s == "SWM_V" || s == "V") src = plane_ptr(c, swm::PL_V + n0);
+    } else if (s == "SWM_U" || s == "U" || s == "U_ND") src = plane_ptr(c, swm::PL_U + n0);   // u_nd = u without a solver
+    else if (s == "SWM_V" || s == "V" || s == "V_ND") src = plane_ptr(c, swm::PL_V + n0);
     else if (s == "SWM_ETA" || s == "ETA") src = plane_ptr(c, swm::PL_E + n0);
     else if (s == "G_U") src = plane_ptr(c, swm::PL_GU + r1);
     else if (s == "G_V") src = plane_ptr(c, swm::PL_GV + r1);
@@ -1650,6 +1697,87 @@ int velpot_t(swm_ctx* c, double omega, double rel_tol, int64_t max_sweeps, bool 
     return host_done ? 0 : 1;
 }
 
+// computeVelocityPotential of the elliptic-solver build (calc_lib.f90:500-525)
+// on the device: B = computeDivergence of u, v at N0 (607-622), then
+// main_ElSolv_SOR (ElSolv_SOR.f90:150-222: Chebyshev omega, stop when
+// sqrt(sum res^2) < epsilon) on chi kept between calls, as calc_lib::chi is,
+// with epsilon = EPS (calc_lib.h:12, the default-kind literal 1e-4, i.e. its
+// single-precision value) and first_guess = (itt == 1).  At most once per
+// step, as chi_computed is reset by advanceCalcLib (calc_lib.f90:198-201).
+// Sweeps in the context's precision.
+template <typename T>
+int elliptic_chi_t(swm_ctx* c) {
+    if (c->chi_ref_itt == c->itt) return 0;
+    const int Nx = c->Nx, Ny = c->Ny;
+    const int nblk = (int)(((long)Ny * ((Nx + 1) / 2) + 255) / 256);
+    const bool f32 = std::is_same<T, float>::value;
+    if (int rc = sor_prepare(c, f32, nblk)) return rc;
+    const size_t bytes = (size_t)c->P * Ny * sizeof(T);
+    if (!c->chi_ref) {
+        if (!(c->chi_ref = dalloc(c, bytes)) || !(c->chi_refB = dalloc(c, bytes))) return fail(c, "out of device memory", 3);
+        HIPCHK(c, zero_sync(c, c->chi_ref, bytes));   // calc_lib::chi starts at 0
+    }
+    const long n = (long)Nx * Ny;
+    hipLaunchKernelGGL((swm::divergence_kernel<T, T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, Nx, Ny,
+                       c->P, (const T*)plane_ptr(c, swm::PL_U + c->cur), (const T*)plane_ptr(c, swm::PL_V + c->cur),
+                       c->mask, (const T*)c->dza, (const T*)c->tab[T_CV], (const T*)c->tab[T_CE],
+                       (T)(c->d.A * c->d.dTheta), (T)c->d.lbc, (T*)c->chi_refB);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemsetAsync(c->sor_done, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->sor_iters, 0, sizeof(long long), c->stream));
+    const float mx = (float)std::max(Nx, Ny);
+    const int64_t max_count = c->itt == 1 ? (int64_t)lrintf(1000.f * mx) : (int64_t)lrintf(10.f * mx);
+    const double eps = (double)1e-4f;
+    int host_done = 0;
+    const int w = f32 ? 1 : 0;
+    if (int rc = sor_sweeps<T>(c, c->chi_ref, c->chi_refB, c->sor_ftab[w], true, 1.0, eps, 0.0, max_count, nblk,
+                               &host_done))
+        return rc;
+    long long it = 0;
+    double nrm = 0.0;
+    HIPCHK(c, copy_sync(c, &it, c->sor_iters, sizeof(it), hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c, &nrm, c->sor_norm, sizeof(nrm), hipMemcpyDeviceToHost));
+    c->chi_ref_iters = host_done ? it : max_count;
+    c->chi_ref_anorm = nrm;
+    c->chi_ref_itt = c->itt;
+    return 0;
+}
+
+int elliptic_chi(swm_ctx* c) { return c->fp32 ? elliptic_chi_t<float>(c) : elliptic_chi_t<double>(c); }
+
+// computeNonDivergentFlowField (calc_lib.f90:548-590) into nd_buf: u - du,
+// v - dv with (du, dv) = computeGradient(chi, eta_grid).
+int elliptic_nd(swm_ctx* c) {
+    if (int rc = elliptic_chi(c)) return rc;
+    const size_t bytes = (size_t)c->P * c->Ny * sizeof(double);
+    for (auto& b : c->nd_buf)
+        if (!b && !(b = (double*)dalloc(c, bytes))) return fail(c, "out of device memory", 3);
+    if (!c->gza) {
+        std::vector<double> g(2 * (size_t)c->Ny);
+        const double A = c->d.A, dL = c->d.dLambda;
+        for (int j = 0; j < c->Ny; ++j) {
+            g[2 * j] = 1.0 / (A * c->cu_host[j] * dL);
+            g[2 * j + 1] = c->d.lbc / (A * c->cu_host[j] * dL);
+        }
+        if (!(c->gza = (double*)dalloc(c, g.size() * 8))) return fail(c, "out of device memory", 3);
+        HIPCHK(c, copy_sync(c, c->gza, g.data(), g.size() * 8, hipMemcpyHostToDevice));
+    }
+    const long n = (long)c->Nx * c->Ny;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (c->fp32)
+        hipLaunchKernelGGL((swm::nondivergent_kernel<float, float>), grid, block, 0, c->stream, c->Nx, c->Ny, c->P,
+                           c->mask, (const float*)plane_ptr(c, swm::PL_U + c->cur),
+                           (const float*)plane_ptr(c, swm::PL_V + c->cur), (const float*)c->chi_ref,
+                           (const double*)c->gza, c->d.lbc, c->d.A * c->d.dTheta, c->nd_buf[0], c->nd_buf[1]);
+    else
+        hipLaunchKernelGGL((swm::nondivergent_kernel<double, double>), grid, block, 0, c->stream, c->Nx, c->Ny, c->P,
+                           c->mask, (const double*)plane_ptr(c, swm::PL_U + c->cur),
+                           (const double*)plane_ptr(c, swm::PL_V + c->cur), (const double*)c->chi_ref,
+                           (const double*)c->gza, c->d.lbc, c->d.A * c->d.dTheta, c->nd_buf[0], c->nd_buf[1]);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
 // FP64 conservation sums of the current state into dout (3 values, copied to
 // the host out): sums_kernel's per-block partials in `partial` (6 per block,
 // at most 1024 blocks), the eta rows' cos(lat) in `ce`.
@@ -1901,6 +2029,15 @@ int swm_gpu_velocity_potential(swm_ctx* c, double omega, double rel_tol, int64_t
     if (c->fp32) return velpot_t<float, float>(c, omega, rel_tol, max_sweeps, reset != 0, iters, rel_res);
     if (fp32_sweeps) return velpot_t<double, float>(c, omega, rel_tol, max_sweeps, reset != 0, iters, rel_res);
     return velpot_t<double, double>(c, omega, rel_tol, max_sweeps, reset != 0, iters, rel_res);
+}
+
+int swm_gpu_elliptic_stats(swm_ctx* c, int64_t* iters, double* anorm) {
+    if (!c || !iters || !anorm) return 1;
+    HipLock lock(g_hip_mu);
+    if (c->chi_ref_itt < 0) return fail(c, "no velocity potential of the elliptic-solver build has been computed", 1);
+    *iters = c->chi_ref_iters;
+    *anorm = c->chi_ref_anorm;
+    return 0;
 }
 
 int swm_gpu_divergence(swm_ctx* c, double* host_div) {

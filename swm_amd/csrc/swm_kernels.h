@@ -1906,14 +1906,46 @@ __device__ __forceinline__ void nsum_add(double& s, double& c, double x) {
     s = t;
 }
 
-// computeStreamfunction (src/calc_lib.f90:727-780) of the master u, v at N0,
-// built without an elliptic solver (u_nd = u, 563-564).  Path integrals in
+// computeStreamfunction (src/calc_lib.f90:727-780) of the master u, v at N0
+// (without an elliptic solver u_nd = u, 563-564).  Path integrals in
 // FP64, each SUM from its first element to its last as the reference's loop
 // runs it.  psi_prep writes the summands: pv = [H_v*]v, with the ocean_v
 // factor in row 1 (H_v*ocean_v*v); pu = [H_u*]u, with ocean_u in column Nx.
+// computeGradient (src/calc_lib.f90:638-648) of the velocity potential on the
+// eta grid: pder_zonal onto the u grid with the points (i, i-1) and
+// pder_meridional onto the v grid with (j, j-1) (pder_zonal3D and
+// pder_meridional3D, 892-998; getOutGrid, 1220-1310), zero where both eta
+// points are land, scaled by the out grid's bc (1 on its ocean points, lbc on
+// its land points).  gza[2j] = 1 / (A cos_u(j) dLambda), gza[2j + 1] =
+// lbc / (A cos_u(j) dLambda), the reference's bc / (A*cos*dLambda).  Then
+// computeNonDivergentFlowField (548-590): u_nd = u - du, v_nd = v - dv, on
+// every point, into und / vnd (FP64).
+template <typename T, typename TS>
+__global__ void __launch_bounds__(256)
+nondivergent_kernel(int Nx, int Ny, int P, const uint8_t* mask, const T* u, const T* v, const TS* chi,
+                    const double* gza, double lbc, double AdT, double* und, double* vnd) {
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (long)Nx * Ny) return;
+    const int j = (int)(idx / Nx), i = (int)(idx - (long)j * Nx);
+    const int im = i == 0 ? Nx - 1 : i - 1, jm = j == 0 ? Ny - 1 : j - 1;
+    const size_t off = (size_t)j * P + i;
+    const uint8_t m = mask[off], mim = mask[(size_t)j * P + im], mjm = mask[(size_t)jm * P + i];
+    double du = 0.0, dv_ = 0.0;
+    if ((mim & OC_E) || (m & OC_E)) du = gza[2 * j + ((m & OC_U) ? 0 : 1)] * ((double)chi[off] - (double)chi[(size_t)j * P + im]);
+    if ((m & OC_E) || (mjm & OC_E)) {
+        const double bc = (m & OC_V) ? 1.0 : lbc;
+        dv_ = bc * ((double)chi[off] - (double)chi[(size_t)jm * P + i]) / AdT;
+    }
+    und[off] = (double)u[off] - du;
+    vnd[off] = (double)v[off] - dv_;
+}
+
+// und / vnd (optional): the non-divergent flow of the elliptic-solver build
+// (computeStreamfunction calls computeNonDivergentFlowField first,
+// src/calc_lib.f90:737), used in place of u, v.
 template <typename T>
 __global__ void __launch_bounds__(256)
-psi_prep(const Statics<T>* sp, const T* u, const T* v, double* pu, double* pv) {
+psi_prep(const Statics<T>* sp, const T* u, const T* v, const double* und, const double* vnd, double* pu, double* pv) {
     const Statics<T>& a = *sp;
     const int Nx = a.Nx, Ny = a.Ny, P = a.P;
     const long n = (long)Nx * Ny;
@@ -1923,7 +1955,7 @@ psi_prep(const Statics<T>* sp, const T* u, const T* v, double* pu, double* pv) {
     const size_t off = (size_t)j * P + i;
     const int m = a.mask[off];
     const double ou = (m & OC_U) ? 1.0 : 0.0, ov = (m & OC_V) ? 1.0 : 0.0;
-    const double uu = (double)u[off], vv = (double)v[off];
+    const double uu = und ? und[off] : (double)u[off], vv = vnd ? vnd[off] : (double)v[off];
     const double hu = (double)fld(a.Hu, off, j, (m & OC_U) != 0), hv = (double)fld(a.Hv, off, j, (m & OC_V) != 0);
     if (a.barotropic) {
         pv[off] = j == 0 ? hv * ov * vv : hv * vv;

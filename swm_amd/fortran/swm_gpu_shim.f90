@@ -15,7 +15,7 @@ module swm_gpu_shim
   integer(c_int32_t), parameter, public :: SWM_FULLY_NONLINEAR = 1, SWM_LINEARISED_MEAN_STATE = 2, &
                                            SWM_LINEARISED_STATE_OF_REST = 3
   integer(c_int32_t), parameter, public :: SWM_LATERAL_MIXING = 1, SWM_QUADRATIC_BOTTOM_FRICTION = 2, &
-                                           SWM_BAROTROPIC = 4, SWM_TAU_SCALE = 8
+                                           SWM_BAROTROPIC = 4, SWM_TAU_SCALE = 8, SWM_ELLIPTIC_SOLVER = 16
   integer(c_int32_t), parameter, public :: SWM_FP64 = 0, SWM_FP32 = 1
   integer(c_int32_t), parameter, public :: SWM_COMP_X = 0, SWM_COMP_Y = 1, SWM_COMP_ETA = 2
   integer(c_int32_t), parameter, public :: SWM_STREAM_WIND = 0, SWM_STREAM_CUSTOM = 1, SWM_STREAM_HEATING = 2
@@ -134,6 +134,18 @@ module swm_gpu_shim
       integer(c_int32_t), value :: tracer_id
       real(c_double), dimension(3), intent(out) :: out
     end function
+    integer(c_int) function swm_gpu_sums_noalloc(ctx, tracer_id, out) bind(C, name="swm_gpu_sums_noalloc")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tracer_id
+      real(c_double), dimension(3), intent(out) :: out
+    end function
+    integer(c_int) function swm_gpu_elliptic_stats(ctx, iters, anorm) bind(C, name="swm_gpu_elliptic_stats")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: iters
+      real(c_double), intent(out) :: anorm
+    end function
     integer(c_int) function swm_gpu_tune(ctx, steps, best_shape, times_ms) bind(C, name="swm_gpu_tune")
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx, times_ms
@@ -153,7 +165,8 @@ module swm_gpu_shim
   public :: swm_gpu_create, swm_gpu_upload_state, swm_gpu_add_tracer, swm_gpu_add_forcing_stream, &
             swm_gpu_step, swm_gpu_itt, &
             swm_gpu_sync, swm_gpu_sor, swm_gpu_velocity_potential, swm_gpu_avg_add, swm_gpu_avg_read, &
-            swm_gpu_sums, swm_gpu_tune, swm_gpu_last_error, swm_gpu_destroy
+            swm_gpu_sums, swm_gpu_sums_noalloc, swm_gpu_elliptic_stats, swm_gpu_tune, swm_gpu_last_error, &
+            swm_gpu_destroy
   public :: gpu_check, gpu_sync_field, gpu_main_ElSolv_SOR, gpu_avg_register, gpu_avg_write
 
 contains

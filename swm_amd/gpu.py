@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(_HERE, "libswmgpu.so")
 ABI_VERSION = 2
 
 FULLY_NONLINEAR, LINEARISED_MEAN_STATE, LINEARISED_STATE_OF_REST = 1, 2, 3
-LATERAL_MIXING, QUADRATIC_BOTTOM_FRICTION, BAROTROPIC, TAU_SCALE = 1, 2, 4, 8
+LATERAL_MIXING, QUADRATIC_BOTTOM_FRICTION, BAROTROPIC, TAU_SCALE, ELLIPTIC_SOLVER = 1, 2, 4, 8, 16
 FP64, FP32 = 0, 1
 
 EXPORTED = [
@@ -36,7 +36,7 @@ EXPORTED = [
     "swm_gpu_batch_remaining", "swm_gpu_batch_step", "swm_gpu_batch_last_timing", "swm_gpu_batch_last_error",
     "swm_gpu_batch_destroy", "swm_gpu_kernel_shape_count", "swm_gpu_set_kernel_shape", "swm_gpu_tune",
     "swm_gpu_set_sor_check_every", "swm_gpu_batch_set_shape", "swm_gpu_batch_tune", "swm_gpu_avg_add",
-    "swm_gpu_avg_read", "swm_gpu_add_forcing_stream", "swm_gpu_sums_noalloc",
+    "swm_gpu_avg_read", "swm_gpu_add_forcing_stream", "swm_gpu_sums_noalloc", "swm_gpu_elliptic_stats",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -112,6 +112,8 @@ def load_library(path: str = LIB_PATH):
     L.swm_gpu_sums.restype = C.c_int
     L.swm_gpu_sums_noalloc.argtypes = [p, C.c_int32, _dp]
     L.swm_gpu_sums_noalloc.restype = C.c_int
+    L.swm_gpu_elliptic_stats.argtypes = [p, C.POINTER(C.c_int64), _dp]
+    L.swm_gpu_elliptic_stats.restype = C.c_int
     L.swm_gpu_last_timing.argtypes = [p, _dp, _dp]
     L.swm_gpu_last_timing.restype = C.c_int
     L.swm_gpu_last_error.argtypes = [p]
@@ -174,6 +176,8 @@ def _flags(s) -> int:
         f |= BAROTROPIC
     if s.tau_scale is not None:
         f |= TAU_SCALE
+    if getattr(s, "elliptic_solver", False):
+        f |= ELLIPTIC_SOLVER
     return f
 
 
@@ -328,6 +332,13 @@ class SwmGpu:
         self._check(self.L.swm_gpu_sync(self._h, name.encode(), int(tracer), out.ctypes.data_as(_dp)),
                     f"swm_gpu_sync({name})")
         return out
+
+    def elliptic_stats(self):
+        """(sweeps, anorm) of the last computeVelocityPotential solve of the
+        elliptic-solver build (swm_gpu_elliptic_stats)."""
+        it, an = C.c_int64(), C.c_double()
+        self._check(self.L.swm_gpu_elliptic_stats(self._h, C.byref(it), C.byref(an)), "swm_gpu_elliptic_stats")
+        return it.value, an.value
 
     def divergence(self) -> np.ndarray:
         out = np.empty((self.s.Ny, self.s.Nx), dtype=np.float64)

@@ -231,6 +231,7 @@ class Setup:
     lateral_mixing: bool = False
     quadratic_friction: bool = False
     barotropic: bool = True
+    elliptic_solver: bool = False     # built with CALC_LIB_ELLIPTIC_SOLVER = ElSolv_SOR (include/calc_lib.h)
     tau_scale: Optional[float] = None
     G: float = f32(9.80665)
     Ah: float = 0.0

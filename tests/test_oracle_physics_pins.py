@@ -287,3 +287,31 @@ def test_mixing_operator_symmetric_and_dissipative(lbc):
     ab, ba, aa = dot(a, Lb), dot(b, La), dot(a, La)
     assert abs(ab - ba) <= 1e-12 * abs(aa)
     assert aa < 0.0 and dot(b, Lb) < 0.0
+
+
+@pytest.mark.parametrize("lbc", [0.0, 2.0])
+def test_nondivergent_flow_divergence_is_the_solver_residual(lbc):
+    """computeNonDivergentFlowField: div(u - grad chi) = div(u) - L chi, with L
+    the SOR operator (ElSolv_SOR.f90:54-62), because the divergence of the
+    gradient is that 5-point operator; on every ocean cell with free slip, and
+    on the cells with ocean faces with no slip (whose land faces carry the bc
+    factor 2 into the gradient).  A converged chi leaves a small divergence."""
+    s = configs.basin(40, 30, 10e3, 10e3, variant=host.FULLY_NONLINEAR, H0=500.0, dt=10.0, seed=3,
+                      land_fraction=0.08, lbc=lbc)
+    o = oracle.Oracle(host.finalize(s))
+    rng = np.random.default_rng(0)
+    u = rng.standard_normal(s.shape) * s.grids["u"].ocean
+    v = rng.standard_normal(s.shape) * s.grids["v"].ocean
+    B = o.divergence(u, v)
+    chi = rng.standard_normal(s.shape)
+    un, vn = o.nondivergent(u, v, chi)
+    d = o.divergence(un, vn)
+    Lc = o.sor_apply(chi)
+    ou, ov, oe = s.grids["u"].ocean == 1, s.grids["v"].ocean == 1, s.grids["eta"].ocean == 1
+    cells = oe if lbc == 0.0 else oe & ou & np.roll(ou, -1, axis=1) & ov & np.roll(ov, -1, axis=0)
+    assert cells.sum() > 800
+    assert np.abs(d - (B - Lc))[cells].max() <= 1e-12 * np.abs(Lc).max()
+    chi_c, it, rr, rc = o.sor_fixed(B, np.zeros(s.shape), 1.8, 1e-12, 20000)
+    assert rc == 0
+    dn = o.divergence(*o.nondivergent(u, v, chi_c))
+    assert np.abs(dn[cells]).max() <= 1e-9 * np.abs(B[cells]).max()
