@@ -136,6 +136,18 @@ int swm_gpu_add_tracer(swm_ctx *c, const swm_tracer_desc *t, int32_t *tracer_id)
 int swm_gpu_add_forcing_stream(swm_ctx *c, int32_t kind, int32_t component, int32_t nrec, double rec_dt,
                                int32_t chunk_size, const double *records, int32_t *stream_id);
 
+/* Replaces DFF_initDynFromFile (src/dynFromFile_module.f90:56-95) for one
+ * DYNFROMFILE dataset: nrec (Nx,Ny) records of u (SWM_COMP_X), v (SWM_COMP_Y)
+ * or eta (SWM_COMP_ETA) spaced rec_dt model seconds, read through a
+ * memoryChunk of chunk_size records (DFF_Nt_chunksize).  At every
+ * model_advance the master fields become ocean * getChunkData(itt*dt) +
+ * SWM field (DFF_advance, 137-160; model.f90:238-259): the tracers are
+ * advected by that flow, and swm_gpu_sync("U"/"V"/"ETA") returns the master
+ * fields.  Streamfunction datasets (FileName_psi) are not taken.  A context
+ * with datasets is stepped alone, on the tile kernels, without graph replay. */
+int swm_gpu_add_dyn_stream(swm_ctx *c, int32_t field, int32_t nrec, double rec_dt, int32_t chunk_size,
+                           const double *records);
+
 /* nsteps iterations of the time loop body of src/model.f90:159-196 without Diag:
  * SWM_timestep (src/swm_module.f90:71-74), TRC_timestep
  * (src/tracer_module.f90:109-123) and model_advance (src/model.f90:238-259). */

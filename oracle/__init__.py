@@ -49,6 +49,8 @@ class _Desc(C.Structure):
         ("n_tdf", C.c_int32), ("tdf_kind", C.c_int32 * MAX_STREAMS), ("tdf_comp", C.c_int32 * MAX_STREAMS),
         ("tdf_nrec", C.c_int32 * MAX_STREAMS), ("tdf_chunk", C.c_int32 * MAX_STREAMS),
         ("tdf_dt", C.c_double * MAX_STREAMS), ("tdf_rec", _dp * MAX_STREAMS),
+        ("n_dff", C.c_int32), ("dff_field", C.c_int32 * 3), ("dff_nrec", C.c_int32 * 3),
+        ("dff_chunk", C.c_int32 * 3), ("dff_dt", C.c_double * 3), ("dff_rec", _dp * 3),
     ]
 
 
@@ -143,6 +145,13 @@ class Oracle:
             d.tdf_kind[n], d.tdf_comp[n] = st.kind, st.comp
             d.tdf_nrec[n], d.tdf_chunk[n], d.tdf_dt[n] = st.records.shape[0], st.chunk_size, st.rec_dt
             d.tdf_rec[n] = _ptr(np.ascontiguousarray(st.records, np.float64), k)
+        dyn = getattr(s, "dyn", [])
+        assert len(dyn) <= 3
+        d.n_dff = len(dyn)
+        for n, st in enumerate(dyn):
+            d.dff_field[n], d.dff_nrec[n], d.dff_chunk[n], d.dff_dt[n] = (st.comp, st.records.shape[0], st.chunk_size,
+                                                                         st.rec_dt)
+            d.dff_rec[n] = _ptr(np.ascontiguousarray(st.records, np.float64), k)
         d.n_tracer = len(s.tracers)
         for n, t in enumerate(s.tracers):
             d.kappa_h[n] = t["kappa_h"]

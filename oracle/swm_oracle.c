@@ -74,6 +74,7 @@ struct orc_model {
     double *TRC_coeff;            /* (12, Nx, Ny), tracer_module.f90:42 */
     /* memory chunks of the time-interpolated forcing streams */
     memchunk mc[ORC_MAX_STREAMS];
+    memchunk dff_mc[3];           /* DYNFROMFILE datasets */
     double *tdf_buf;
     /* SOR, ElSolv_SOR.f90:25-32 */
     double *sor_c;                /* (Nx, Ny, 5) */
@@ -798,9 +799,24 @@ static void swm_advance(orc_model *m) {
     forcing_update(m);
 }
 
+/* DFF_advance, dynFromFile_module.f90:137-160: the master field += ocean *
+ * getChunkData(chunk, itt*dt) for each dataset given (u, v, eta). */
+static void dff_advance(orc_model *m) {
+    size_t n = (size_t)m->Nx * m->Ny;
+    const double model_time = (double)m->itt * m->d.dt;
+    for (int s = 0; s < m->d.n_dff; ++s) {
+        const int f = m->d.dff_field[s];
+        double *dst = f == ORC_COMP_X ? m->u : (f == ORC_COMP_Y ? m->v : m->eta);
+        const int8_t *oc = m->ocean[f == ORC_COMP_X ? ORC_GRID_U : (f == ORC_COMP_Y ? ORC_GRID_V : ORC_GRID_ETA)];
+        mc_get(&m->dff_mc[s], model_time, m->tdf_buf);
+        for (size_t k = 0; k < n; ++k) dst[k] = dst[k] + (double)oc[k] * m->tdf_buf[k];
+    }
+}
+
 static void model_advance(orc_model *m) {
     size_t n = (size_t)m->Nx * m->Ny;
     for (size_t k = 0; k < n; ++k) { m->u[k] = 0.0; m->v[k] = 0.0; m->eta[k] = 0.0; }
+    dff_advance(m);
     swm_advance(m);
     for (int t = 0; t < m->ntrc; ++t) trc_advance(m, &m->trc[t]);
 }
@@ -860,6 +876,11 @@ orc_model *orc_create(const orc_desc *d) {
     for (int s = 0; s < d->n_tdf; ++s)   /* SWM_forcing_initStream -> initMemChunk, 593-607 */
         mc_init(&m->mc[s], d->tdf_rec[s], n, d->tdf_nrec[s], d->tdf_chunk[s], d->tdf_dt[s]);
     m->tdf_buf = zalloc(n);
+    /* DFF_initDynFromFile (model.f90:108-113, before SWM_initSWM): initMemChunk
+     * of each dataset and DFF_advance at itt = 0 */
+    for (int s = 0; s < d->n_dff; ++s)
+        mc_init(&m->dff_mc[s], d->dff_rec[s], n, d->dff_nrec[s], d->dff_chunk[s], d->dff_dt[s]);
+    dff_advance(m);
     forcing_update(m);
     /* SWM_initialConditions, swm_module.f90:156-181 */
     for (size_t k = 0; k < n; ++k) {
@@ -949,7 +970,10 @@ int orc_get(const orc_model *m, const char *name, int tracer, double *out) {
     size_t n = (size_t)m->Nx * m->Ny;
     const double *src = NULL;
     const tracer_t *t = (tracer >= 0 && tracer < m->ntrc) ? &m->trc[tracer] : NULL;
-    if (!strcmp(name, "u")) src = m->SWM_u[0];
+    if (!strcmp(name, "U")) src = m->u;            /* master fields (vars_module u, v, eta at N0) */
+    else if (!strcmp(name, "V")) src = m->v;
+    else if (!strcmp(name, "ETA")) src = m->eta;
+    else if (!strcmp(name, "u")) src = m->SWM_u[0];
     else if (!strcmp(name, "v")) src = m->SWM_v[0];
     else if (!strcmp(name, "eta")) src = m->SWM_eta[0];
     else if (!strcmp(name, "G_u")) src = m->G_u[NG - 1];
@@ -1130,6 +1154,7 @@ void orc_destroy(orc_model *m) {
     }
     free(m->TRC_coeff); free(m->sor_c); free(m->i_oe);
     for (int s = 0; s < m->d.n_tdf; ++s) { free(m->mc[s].var); free(m->mc[s].time); }
+    for (int s = 0; s < m->d.n_dff; ++s) { free(m->dff_mc[s].var); free(m->dff_mc[s].time); }
     free(m->tdf_buf);
     free(m);
 }

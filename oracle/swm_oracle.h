@@ -68,6 +68,14 @@ typedef struct orc_desc {
     int32_t tdf_chunk[ORC_MAX_STREAMS];     /* namelist chunksize */
     double tdf_dt[ORC_MAX_STREAMS];         /* record spacing in model seconds (getDt) */
     const double *tdf_rec[ORC_MAX_STREAMS]; /* nrec (Nx,Ny) records back to back */
+    /* DYNFROMFILE datasets (dynFromFile_module.f90): the field each adds to
+     * the master fields at model_advance (ORC_COMP_X = u, _Y = v, _ETA = eta) */
+    int32_t n_dff;
+    int32_t dff_field[3];
+    int32_t dff_nrec[3];
+    int32_t dff_chunk[3];                   /* namelist DFF_Nt_chunksize */
+    double dff_dt[3];
+    const double *dff_rec[3];
 } orc_desc;
 
 typedef struct orc_model orc_model;

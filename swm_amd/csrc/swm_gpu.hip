@@ -165,6 +165,9 @@ struct FStream {
     ChunkIdx idx;
 };
 
+template <typename T>
+using SStepArr = swm::SStep<T>[2];
+
 struct Tracer {
     double kappa = 0;
     bool cbuf_valid = false;
@@ -201,6 +204,7 @@ struct swm_ctx {
     double ab[5][5] = {};
     std::vector<Tracer> trc;
     std::vector<FStream> streams;   // time-interpolated forcing streams, in processing order
+    FStream dyn[3];                 // DYNFROMFILE datasets of u, v, eta (rec null when absent)
     int tile = 0;
     // Steady steps replayed from captured six-step graphs (SWM_GPU_GRAPH=1).
     // Off by default: on MI355X direct launches measured faster at every grid
@@ -643,6 +647,8 @@ int write_statics_t(swm_ctx* c) {
         a.scomp[n] = c->streams[n].comp;
         a.srec[n] = (const T*)c->streams[n].rec;
     }
+    a.dyn_u = (const T*)c->dyn[0].rec;
+    a.dyn_v = (const T*)c->dyn[1].rec;
     static_assert(swm::RT_N >= swm::RT_TRC + swm::RT_PER_TRC * kMaxTracers, "rowtab has room for every tracer");
     if (!c->statics && !(c->statics = dalloc(c, sizeof(Statics<T>)))) return fail(c, "out of device memory", 3);
     if (!c->rowtab && !(c->rowtab = dalloc(c, (size_t)c->Ny * swm::RT_N * sizeof(T)))) return fail(c, "out of device memory", 3);
@@ -845,6 +851,25 @@ void set_streams(swm_ctx* c, Dyn<T>& d, double tf) {
     }
 }
 
+// Place of the DYNFROMFILE datasets at model time tf*dt (DFF_advance is
+// called with itt*dt at every model_advance; the step that starts from itt
+// reads the master fields of that advance).
+template <typename T>
+void set_dyn(swm_ctx* c, SStepArr<T>& ds, double tf) {
+    for (int f = 0; f < 2; ++f) {
+        if (!c->dyn[f].rec) continue;
+        int64_t r0 = 0, r1 = 0;
+        bool exact = true;
+        double a = 0.0, b = 1.0;
+        c->dyn[f].idx.query(tf * c->d.dt, r0, r1, exact, a, b);
+        ds[f].o0 = r0 * c->plane;
+        ds[f].o1 = r1 * c->plane;
+        ds[f].exact = exact ? 1 : 0;
+        ds[f].a = (T)a;
+        ds[f].b = (T)b;
+    }
+}
+
 template <typename T>
 int step_one(swm_ctx* c) {
     c->itt += 1;
@@ -857,6 +882,7 @@ int step_one(swm_ctx* c) {
     // the forcing used by step itt was set by SWM_forcing_update at itt-1
     set_osc(c, d, (double)(c->itt - 1));
     set_streams(c, d, (double)(c->itt - 1));
+    set_dyn<T>(c, d.ds, (double)(c->itt - 1));
     int rc = launch_step_t<T, false>(c, d, c->tile);
     if (rc) return rc;
     // model_advance: N0 <- N0p1, G(1) <- G(2), G(2) <- G(3)
@@ -933,7 +959,9 @@ int capture_graph(swm_ctx* c, int cur, int ring) {
 // time-interpolated stream, whose per-step arguments change every step):
 // the launch arguments repeat with the period 6 of the ping-pong x ring roles,
 // so six steps are captured once per phase and replayed as one graph.
-bool steady_ok(const swm_ctx* c) { return c->use_graph && !c->has_osc && c->streams.empty(); }
+bool steady_ok(const swm_ctx* c) {
+    return c->use_graph && !c->has_osc && c->streams.empty() && !c->dyn[0].rec && !c->dyn[1].rec && !c->dyn[2].rec;
+}
 
 // Build, before any step is enqueued, every graph that step_many(nsteps) will
 // replay, so that the host-side capture and instantiation stay outside the
@@ -1379,6 +1407,29 @@ int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_o
         } else {
             return fail(c, "unknown tracer field " + s, 1);
         }
+    } else if ((s == "U" || s == "V" || s == "ETA") && c->dyn[s == "U" ? 0 : (s == "V" ? 1 : 2)].rec) {
+        // master field = ocean * DYNFROMFILE data at itt*dt + SWM field (model_advance)
+        const int f = s == "U" ? 0 : (s == "V" ? 1 : 2);
+        const int pl = f == 0 ? swm::PL_U : (f == 1 ? swm::PL_V : swm::PL_E);
+        const uint8_t bit = f == 0 ? swm::OC_U : (f == 1 ? swm::OC_V : swm::OC_E);
+        int64_t r0 = 0, r1 = 0;
+        bool exact = true;
+        double qa = 0.0, qb = 1.0;
+        c->dyn[f].idx.query((double)c->itt * c->d.dt, r0, r1, exact, qa, qb);
+        const long n = (long)c->Nx * c->Ny;
+        if (c->fp32) {
+            swm::SStep<float> st{r0 * c->plane, r1 * c->plane, exact ? 1 : 0, (float)qa, (float)qb};
+            hipLaunchKernelGGL(swm::master_kernel<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                               c->Nx, c->Ny, c->P, c->mask, bit, (const float*)c->dyn[f].rec, st,
+                               (const float*)plane_ptr(c, pl + n0), (float*)c->scratch);
+        } else {
+            swm::SStep<double> st{r0 * c->plane, r1 * c->plane, exact ? 1 : 0, qa, qb};
+            hipLaunchKernelGGL(swm::master_kernel<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                               c->Nx, c->Ny, c->P, c->mask, bit, (const double*)c->dyn[f].rec, st,
+                               (const double*)plane_ptr(c, pl + n0), (double*)c->scratch);
+        }
+        HIPCHK(c, hipGetLastError());
+        src = c->scratch;
     } else if (s == "SWM_U" || s == "U" || s == "U_ND") src = plane_ptr(c, swm::PL_U + n0);   // u_nd = u without a solver
     else if (s == "SWM_V" || s == "V" || s == "V_ND") src = plane_ptr(c, swm::PL_V + n0);
     else if (s == "SWM_ETA" || s == "ETA") src = plane_ptr(c, swm::PL_E + n0);
@@ -1863,6 +1914,8 @@ int swm_gpu_set_kernel_shape(swm_ctx* c, int32_t shape) {
     if (shape < 0 || shape >= kNumTiles) return fail(c, "no such kernel shape", 1);
     if (c->d.variant == SWM_LINEARISED_MEAN_STATE && kTiles[shape].march)
         return fail(c, "LINEARISED_MEAN_STATE runs on the LDS tile kernels (shapes 0-4, 11, 12)", 4);
+    if ((c->dyn[0].rec || c->dyn[1].rec) && kTiles[shape].march == 1)
+        return fail(c, "DYNFROMFILE velocities are read by the tile kernels (shapes 0-4, 9-12)", 4);
     HipLock lock(g_hip_mu);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     drop_graphs(c);
@@ -1907,6 +1960,39 @@ int swm_gpu_add_forcing_stream(swm_ctx* c, int32_t kind, int32_t component, int3
     return write_statics(c);
 }
 
+int swm_gpu_add_dyn_stream(swm_ctx* c, int32_t field, int32_t nrec, double rec_dt, int32_t chunk_size,
+                           const double* records) {
+    if (!c || !records) return 1;
+    HipLock lock(g_hip_mu);
+    if (int rc = member_refresh(c)) return rc;
+    if (c->batch) return fail(c, "context is attached to a batch; add datasets before attaching", 1);
+    if (field < SWM_COMP_X || field > SWM_COMP_ETA) return fail(c, "unknown field (SWM_COMP_X, _Y, _ETA)", 1);
+    if (c->dyn[field].rec) return fail(c, "one DYNFROMFILE dataset per field", 1);
+    if (nrec < 1) return fail(c, "a dataset needs at least one record", 1);
+    if (nrec > 1 && chunk_size < 2) return fail(c, "a time-dependent dataset needs chunk_size >= 2", 1);
+    if (nrec > 1 && !(rec_dt > 0.0)) return fail(c, "record spacing must be positive", 1);
+    if (kTiles[c->tile].march == 1 && field != SWM_COMP_ETA) c->tile = 11;   // the tile kernels read the velocities
+    FStream st;
+    st.kind = -1;
+    st.comp = field;
+    st.rec = dalloc(c, (size_t)nrec * c->plane * c->esz);
+    if (!st.rec) return fail(c, "out of device memory for the dataset records", 3);
+    HIPCHK(c, zero_sync(c, st.rec, (size_t)nrec * c->plane * c->esz));
+    for (int32_t r = 0; r < nrec; ++r) {
+        int rc = upload_grid(c, (char*)st.rec + (size_t)r * c->plane * c->esz, records + (size_t)r * c->Nx * c->Ny);
+        if (rc) return rc;
+    }
+    st.idx.init(nrec, chunk_size, rec_dt);
+    // DFF_initDynFromFile's DFF_advance at itt = 0 (model.f90:108-113) reads the first time
+    int64_t r0, r1;
+    bool ex;
+    double qa, qb;
+    st.idx.query(0.0, r0, r1, ex, qa, qb);
+    c->dyn[field] = std::move(st);
+    drop_graphs(c);
+    return write_statics(c);
+}
+
 int swm_gpu_tune(swm_ctx* c, int32_t steps, int32_t* best_shape, double* times_ms) {
     if (!c || !best_shape) return 1;
     if (c->batch) return fail(c, "context is attached to a batch; tune the batch instead", 1);
@@ -1936,8 +2022,9 @@ int swm_gpu_tune(swm_ctx* c, int32_t steps, int32_t* best_shape, double* times_m
     if (hipMemcpyAsync(snap, c->state, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) rc = 3;
     c->use_graph = false;   // time the kernels, not graph capture
     for (int k = 0; k < kNumTiles && !rc; ++k) {
-        if (c->d.variant == SWM_LINEARISED_MEAN_STATE && kTiles[k].march) {
-            if (times_ms) times_ms[k] = -1.0;   // not available for this variant
+        if ((c->d.variant == SWM_LINEARISED_MEAN_STATE && kTiles[k].march) ||
+            ((c->dyn[0].rec || c->dyn[1].rec) && kTiles[k].march == 1)) {
+            if (times_ms) times_ms[k] = -1.0;   // not available for this variant / these datasets
             continue;
         }
         c->tile = k;
@@ -2303,8 +2390,9 @@ int swm_gpu_batch_attach(swm_batch* b, swm_ctx* c, int64_t nsteps) {
     if (c->has_osc) return bfail(b, "oscillating forcing needs per-step host phases; step such a context alone", 4);
     if (c->d.variant == SWM_LINEARISED_MEAN_STATE)
         return bfail(b, "LINEARISED_MEAN_STATE contexts are stepped alone", 4);
-    if (!c->streams.empty())
-        return bfail(b, "time-interpolated forcing streams need per-step host arguments; step such a context alone", 4);
+    if (!c->streams.empty() || c->dyn[0].rec || c->dyn[1].rec || c->dyn[2].rec)
+        return bfail(b, "time-interpolated forcing or DYNFROMFILE datasets need per-step host arguments; step such a "
+                        "context alone", 4);
     if (!c->avgs.empty()) return bfail(b, "averaging buffers are kept by contexts stepped alone", 4);
     if (b->fp32 >= 0 && (b->fp32 != (int)c->fp32 || b->variant != c->d.variant))
         return bfail(b, "every member of a batch has the same precision and variant", 4);

@@ -85,6 +85,8 @@ struct Statics {
     int nstream, sdiv;   // sdiv: windstress divides by RHO0 * D_face (BAROTROPIC or FULLY_NONLINEAR)
     int skind[4], scomp[4];
     const T* srec[4];    // records of each stream, one plane apart
+    // DYNFROMFILE datasets of u, v (records, one plane apart; null when absent)
+    const T *dyn_u, *dyn_v;
     // diagnostic outputs (materialisation kernel only)
     T* diag[14];      // D Dh Du Dv zeta Pot EDens MU MV F_x F_y F_eta latmix_u latmix_v
     T* tdiag[2][3];   // C uhc vhc per tracer
@@ -163,7 +165,18 @@ struct Dyn {
     T dtc1, dtc2, dtc3;  // dt*ab_coeff(ti, tstep)
     T oc[3], os[3];      // cos/sin(omega*itt*dt) of the oscillating streams (x, y, eta)
     SStep<T> ss[4];      // time-interpolated forcing streams
+    SStep<T> ds[2];      // DYNFROMFILE u, v datasets at the master fields' time
 };
+
+// A DYNFROMFILE dataset at element off: getChunkData interpolated in time as
+// the forcing streams are (src/memchunk_module.f90:214-243).
+template <typename T>
+__device__ __forceinline__ T dyn_val(const T* r, const SStep<T>& st, size_t off) {
+    const T y0 = ((const __attribute__((address_space(1))) T*)(r + st.o0))[off];
+    if (st.exact) return y0;
+    const T y1 = ((const __attribute__((address_space(1))) T*)(r + st.o1))[off];
+    return y0 + (st.a * (y1 - y0)) / st.b;
+}
 
 // weight = 1 / max(sum of masks, 1) of the *_noland interpolators
 // (src/calc_lib.f90:259-262, 326-332): the exact values of those divisions.
@@ -660,7 +673,8 @@ __device__ __forceinline__ void tile_stage(const Dyn<T>& d, const int t, const R
 
 // Phases 2 and 3 of tile t, after its LDS stage and a barrier: derived fields,
 // tendencies, AB integration and stores.  pg/pt: the outputs' AB history.
-template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, typename L, typename G, typename GT, typename GE>
+template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, bool DYN = true, typename L, typename G,
+          typename GT, typename GE>
 __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& lds, const G& pg, const GT& pt,
                                              const GE& pe) {
     constexpr int NT = 256;
@@ -1025,9 +1039,21 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             T* const tb = st + (PL_TRC + PL_PER_TRC * n) * pl;
             const T CH = s_CH[n][k], CHe = s_CH[n][ke], CHw = s_CH[n][kw], CHn = s_CH[n][kn], CHs = s_CH[n][ks];
             const T C = s_C[n][k], Ce = s_C[n][ke], Cw = s_C[n][kw], Cn = s_C[n][kn], Cs = s_C[n][ks];
-            // tracer velocities are the master fields u = 0 + SWM_u (model_advance)
-            const T um = T(0) + s_u[k], ume = T(0) + s_u[ke], vm = T(0) + s_v[k], vmn = T(0) + s_v[kn];
+            // tracer velocities are the master fields (model_advance,
+            // src/model.f90:238-259): u = 0, += ocean_u * DYNFROMFILE data
+            // (DFF_advance, src/dynFromFile_module.f90:137-160), += SWM_u
+            T um = T(0) + s_u[k], ume = T(0) + s_u[ke], vm = T(0) + s_v[k], vmn = T(0) + s_v[kn];
             const uint8_t me = s_m[ke], mw = s_m[kw], mn = s_m[kn], ms = s_m[ks];
+            if (DYN && a.dyn_u) {
+                const size_t offe = (size_t)j * P + (i + 1 == Nx ? 0 : i + 1);
+                um = (T(0) + T(ou) * dyn_val(a.dyn_u, d.ds[0], off)) + s_u[k];
+                ume = (T(0) + T(OU(me)) * dyn_val(a.dyn_u, d.ds[0], offe)) + s_u[ke];
+            }
+            if (DYN && a.dyn_v) {
+                const size_t offn = (size_t)jp * P + i;
+                vm = (T(0) + T(ov) * dyn_val(a.dyn_v, d.ds[1], off)) + s_v[k];
+                vmn = (T(0) + T(OV(mn)) * dyn_val(a.dyn_v, d.ds[1], offn)) + s_v[kn];
+            }
             const int oee = OE(me), oew = OE(mw), oen = OE(mn), oes = OE(ms);
             const int oue = OU(me), ovn = OV(mn);
             T diff = T(0);
@@ -1080,7 +1106,9 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
 // that burst over the barrier and phase 2 (-6 % at 1024^2).  With many rounds
 // the blocks drift apart and the early prefetch hides more latency (+3 % at
 // 2048^2 for the late order).
-template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, bool LATE = false, typename L>
+// DYN: the tracer may read DYNFROMFILE datasets (contexts that step alone;
+// the batched kernel is built without that branch).
+template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, bool LATE = false, bool DYN = true, typename L>
 __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) {
     using S = TileShapeC<TX, TY>;
     TileRegs<T, S::NE, S::NO, (NTRC > 0 ? NTRC : 1), RowTabC<TX, TY, NTRC>::NQ> R;
@@ -1088,7 +1116,7 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
     tile_stage<T, VAR, NTRC, TX, TY, DIAG>(d, t, R, lds);
     if (LATE) tile_load_hist<T, VAR, NTRC, TX, TY, DIAG>(d, t, R);
     __syncthreads();
-    tile_compute<T, VAR, NTRC, TX, TY, DIAG>(d, t, lds, R.g, R.gt, R.eo);
+    tile_compute<T, VAR, NTRC, TX, TY, DIAG, DYN>(d, t, lds, R.g, R.gt, R.eo);
 }
 
 template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, bool LATE = false>
@@ -1193,14 +1221,14 @@ batch_kernel(const Member<T>* __restrict__ mem, const MemberStep* __restrict__ s
     d.dtc3 = m.dtc[tstep - 1][2];
     const int t = w - m.wbase;
     if constexpr (NTRC_MAX == 0) {
-        tile_body<T, VAR, 0, TX, TY, false>(d, t, lds);
+        tile_body<T, VAR, 0, TX, TY, false, false, false>(d, t, lds);
     } else if constexpr (NTRC_MAX == 1) {
-        if (m.ntr == 0) tile_body<T, VAR, 0, TX, TY, false>(d, t, lds);
-        else tile_body<T, VAR, 1, TX, TY, false>(d, t, lds);
+        if (m.ntr == 0) tile_body<T, VAR, 0, TX, TY, false, false, false>(d, t, lds);
+        else tile_body<T, VAR, 1, TX, TY, false, false, false>(d, t, lds);
     } else {
-        if (m.ntr == 0) tile_body<T, VAR, 0, TX, TY, false>(d, t, lds);
-        else if (m.ntr == 1) tile_body<T, VAR, 1, TX, TY, false>(d, t, lds);
-        else tile_body<T, VAR, 2, TX, TY, false>(d, t, lds);
+        if (m.ntr == 0) tile_body<T, VAR, 0, TX, TY, false, false, false>(d, t, lds);
+        else if (m.ntr == 1) tile_body<T, VAR, 1, TX, TY, false, false, false>(d, t, lds);
+        else tile_body<T, VAR, 2, TX, TY, false, false, false>(d, t, lds);
     }
 }
 
@@ -1833,6 +1861,21 @@ divergence_kernel(int Nx, int Ny, int P, const T* u, const T* v, const uint8_t* 
         zb = bc * (cv[jp] * v[(size_t)jp * P + i] - cv[j] * v[off]) / AdT;
     }
     div[off] = (TO)(za + zb / ce[j]);
+}
+
+// A master field after model_advance (src/model.f90:238-259): 0, plus the
+// grid's ocean mask times the DYNFROMFILE dataset (DFF_advance,
+// src/dynFromFile_module.f90:137-160), plus the shallow-water field at N0.
+template <typename T>
+__global__ void __launch_bounds__(256)
+master_kernel(int Nx, int Ny, int P, const uint8_t* mask, uint8_t bit, const T* rec, SStep<T> st, const T* swm,
+              T* out) {
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (long)Nx * Ny) return;
+    const int j = (int)(idx / Nx), i = (int)(idx - (long)j * Nx);
+    const size_t off = (size_t)j * P + i;
+    const T o = (mask[off] & bit) ? T(1) : T(0);
+    out[off] = (T(0) + o * dyn_val(rec, st, off)) + swm[off];
 }
 
 // sum over ocean eta points of B^2 in FP64, fixed order: per-block partials

@@ -37,6 +37,7 @@ EXPORTED = [
     "swm_gpu_batch_destroy", "swm_gpu_kernel_shape_count", "swm_gpu_set_kernel_shape", "swm_gpu_tune",
     "swm_gpu_set_sor_check_every", "swm_gpu_batch_set_shape", "swm_gpu_batch_tune", "swm_gpu_avg_add",
     "swm_gpu_avg_read", "swm_gpu_add_forcing_stream", "swm_gpu_sums_noalloc", "swm_gpu_elliptic_stats",
+    "swm_gpu_add_dyn_stream",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -92,6 +93,8 @@ def load_library(path: str = LIB_PATH):
     L.swm_gpu_add_forcing_stream.argtypes = [p, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, _dp,
                                              C.POINTER(C.c_int32)]
     L.swm_gpu_add_forcing_stream.restype = C.c_int
+    L.swm_gpu_add_dyn_stream.argtypes = [p, C.c_int32, C.c_int32, C.c_double, C.c_int32, _dp]
+    L.swm_gpu_add_dyn_stream.restype = C.c_int
     L.swm_gpu_step.argtypes = [p, C.c_int64]
     L.swm_gpu_step.restype = C.c_int
     L.swm_gpu_itt.argtypes = [p]
@@ -249,6 +252,8 @@ class SwmGpu:
         self._keep = []   # inputs are copied to the device during create
         for st in getattr(s, "streams", []):
             self.add_forcing_stream(st.kind, st.comp, st.records, st.rec_dt, st.chunk_size)
+        for st in getattr(s, "dyn", []):
+            self.add_dyn_stream(st.comp, st.records, st.rec_dt, st.chunk_size)
 
     # -- error handling -------------------------------------------------------
     def _check(self, rc: int, what: str):
@@ -300,6 +305,17 @@ class SwmGpu:
                                                       int(chunk_size), rec.ctypes.data_as(_dp), C.byref(sid)),
                     "swm_gpu_add_forcing_stream")
         return sid.value
+
+    def add_dyn_stream(self, comp: int, records, rec_dt: float, chunk_size: int = 100):
+        """A DYNFROMFILE dataset (swm_gpu_add_dyn_stream): records (nrec, Ny, Nx)
+        of u, v or eta spaced rec_dt model seconds."""
+        rec = np.ascontiguousarray(records, dtype=np.float64)
+        if rec.ndim == 2:
+            rec = rec[None]
+        if rec.shape[1:] != (self.s.Ny, self.s.Nx):
+            raise SwmGpuError("records must be (nrec, Ny, Nx)")
+        self._check(self.L.swm_gpu_add_dyn_stream(self._h, int(comp), rec.shape[0], float(rec_dt), int(chunk_size),
+                                                  rec.ctypes.data_as(_dp)), "swm_gpu_add_dyn_stream")
 
     def init_from_setup(self):
         """SWM_initialConditions + TRC_initTracer hand-over for a fresh run."""

@@ -256,6 +256,7 @@ class Setup:
     wind: list = field(default_factory=list)        # [(comp, tau)] re-applied every step
     osc: list = field(default_factory=list)         # [(comp, re, im, omega)]
     streams: list = field(default_factory=list)     # [ForcingStream] time-interpolated, after wind and osc
+    dyn: list = field(default_factory=list)         # [ForcingStream] DYNFROMFILE datasets (comp: u, v, eta)
     u_bs: np.ndarray = None
     v_bs: np.ndarray = None
     zeta_bs: np.ndarray = None
@@ -451,6 +452,28 @@ def add_oscillation(s: Setup, comp: int, re: np.ndarray, im: np.ndarray, omega: 
     if s.streams:
         raise NotImplementedError("oscillating streams are applied before time-interpolated streams; list them first")
     s.osc.append((comp, np.asarray(re, dtype=np.float64), np.asarray(im, dtype=np.float64), float(omega)))
+
+
+DEF_NT_CHUNKSIZE = 100          # DFF_Nt_chunksize default, include/io.h:12
+STREAM_DYN = 3                  # kind of a DYNFROMFILE dataset
+
+
+def add_dyn_from_file(s: Setup, comp: int, records: np.ndarray, rec_dt: float,
+                      chunk_size: int = DEF_NT_CHUNKSIZE) -> None:
+    """A DYNFROMFILE dataset (DFF_initDynFromFile, dynFromFile_module.f90:56-95):
+    nrec (Ny, Nx) records of u (COMP_X), v (COMP_Y) or eta (COMP_ETA), spaced
+    rec_dt model seconds, added times the grid's ocean mask to the master field
+    at every model_advance (DFF_advance, 137-160), i.e. to the flow the tracer
+    is advected by and to the U, V, ETA register fields."""
+    rec = np.asarray(records, dtype=np.float64)
+    if rec.ndim == 2:
+        rec = rec[None]
+    if rec.shape[1:] != s.shape:
+        raise ValueError(f"records must be (nrec, {s.Ny}, {s.Nx})")
+    if any(d.comp == comp for d in s.dyn):
+        raise ValueError("one dataset per field")
+    s.dyn.append(ForcingStream(kind=STREAM_DYN, comp=comp, records=rec, rec_dt=float(rec_dt),
+                               chunk_size=int(chunk_size)))
 
 
 def add_forcing_stream(s: Setup, kind: int, comp: int, records: np.ndarray, rec_dt: float,
