@@ -447,6 +447,47 @@ int download_grid(swm_ctx* c, double* dst, const void* src) {
     return 0;
 }
 
+// bfloat16 low part of the FP32 state, rounded to nearest even (the step
+// kernels' lo_enc / lo_dec)
+uint16_t lo_enc_host(float x) {
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    return (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+double lo_dec_host(uint16_t h) {
+    const uint32_t b = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+// A state field: FP64 as is; FP32 as the pair hi = (float)x in the field's
+// plane and lo = x - hi (bfloat16) in its low-part plane.
+int upload_state_grid(swm_ctx* c, void* dst, void* lo_dst, const double* src) {
+    if (!c->fp32) return upload_grid(c, dst, src);
+    const size_t n = (size_t)c->Nx * c->Ny;
+    std::vector<float> f(n);
+    std::vector<uint16_t> l(n);
+    for (size_t k = 0; k < n; ++k) {
+        f[k] = (float)src[k];
+        l[k] = std::isfinite(f[k]) ? lo_enc_host((float)(src[k] - (double)f[k])) : (uint16_t)0;
+    }
+    HIPCHK(c, copy2d_sync(c, dst, c->P * 4, f.data(), c->Nx * 4, c->Nx * 4, c->Ny, hipMemcpyHostToDevice));
+    HIPCHK(c, copy2d_sync(c, lo_dst, c->P * 2, l.data(), c->Nx * 2, c->Nx * 2, c->Ny, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int download_state_grid(swm_ctx* c, double* dst, const void* src, const void* lo_src) {
+    if (!c->fp32 || !lo_src) return download_grid(c, dst, src);
+    const size_t n = (size_t)c->Nx * c->Ny;
+    std::vector<float> f(n);
+    std::vector<uint16_t> l(n);
+    HIPCHK(c, copy2d_sync(c, f.data(), c->Nx * 4, src, c->P * 4, c->Nx * 4, c->Ny, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy2d_sync(c, l.data(), c->Nx * 2, lo_src, c->P * 2, c->Nx * 2, c->Ny, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < n; ++k) dst[k] = (double)f[k] + lo_dec_host(l[k]);
+    return 0;
+}
+
 void* alloc_grid(swm_ctx* c) {
     void* p = dalloc(c, (size_t)c->Ny * c->P * c->esz);
     if (p && zero_sync(c, p, (size_t)c->Ny * c->P * c->esz) != hipSuccess) return nullptr;
@@ -460,6 +501,19 @@ bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) 
 // Upload a static field.  NULL -> the constant default on every point.  If each
 // row has one value on the mask's ocean points and one on its land points the
 // field is stored as two values per row, else as a full pitched array.
+// The implicit damping factor impl = 1 + dt*gamma as the step kernels take
+// it: FP64 contexts hold impl and divide by it; FP32 contexts hold
+// kap = 1 - 1/impl and form x - x*kap, which keeps the drag to float's
+// relative precision (impl itself, rounded to float, would keep only the
+// few bits of dt*gamma that lie above ulp(1)).  buf holds the FP32 values.
+const double* damp_static(swm_ctx* c, const double* impl, std::vector<double>& buf) {
+    if (!c->fp32 || !impl) return impl;
+    const size_t n = (size_t)c->Nx * c->Ny;
+    buf.resize(n);
+    for (size_t k = 0; k < n; ++k) buf[k] = (impl[k] - 1.0) / impl[k];
+    return buf.data();
+}
+
 int upload_field(swm_ctx* c, FieldDev* out, const double* h, double dflt, const int8_t* mask) {
     const int Nx = c->Nx, Ny = c->Ny;
     std::vector<double> rows(2 * (size_t)Ny, dflt);
@@ -1162,10 +1216,14 @@ static int create_impl(swm_ctx* c, const swm_desc* d) {
     }
     const int8_t *mu = c->hmask[1].data(), *mv = c->hmask[2].data(), *me = c->hmask[3].data();
     int rc = 0;
+    std::vector<double> kap[3];
+    const double* imp[3] = {damp_static(c, d->impl_u, kap[0]), damp_static(c, d->impl_v, kap[1]),
+                            damp_static(c, d->impl_eta, kap[2])};
+    const double idflt = c->fp32 ? 0.0 : 1.0;
     struct { int id; const double* h; double dflt; const int8_t* m; } ups[] = {
         {F_HE, d->grid[SWM_GRID_ETA].H, 0.0, me}, {F_HU, d->grid[SWM_GRID_U].H, 0.0, mu},
-        {F_HV, d->grid[SWM_GRID_V].H, 0.0, mv},   {F_IMU, d->impl_u, 1.0, mu},
-        {F_IMV, d->impl_v, 1.0, mv},              {F_IME, d->impl_eta, 1.0, me},
+        {F_HV, d->grid[SWM_GRID_V].H, 0.0, mv},   {F_IMU, imp[0], idflt, mu},
+        {F_IMV, imp[1], idflt, mv},               {F_IME, imp[2], idflt, me},
         {F_GSU, d->gamma_sq_u, 0.0, mu},          {F_GSV, d->gamma_sq_v, 0.0, mv},
         {F_FXC, d->F_x_const, 0.0, mu},           {F_FYC, d->F_y_const, 0.0, mv},
         {F_FEC, d->F_eta_const, 0.0, me},         {F_TAUX, d->tau_x, 0.0, mu},
@@ -1206,8 +1264,9 @@ int swm_gpu_upload_state(swm_ctx* c, const double* u, const double* v, const dou
     c->cur = 0;
     c->ring = 0;
     int rc;
-    if ((rc = upload_grid(c, plane_ptr(c, swm::PL_U), u)) || (rc = upload_grid(c, plane_ptr(c, swm::PL_V), v)) ||
-        (rc = upload_grid(c, plane_ptr(c, swm::PL_E), eta)))
+    if ((rc = upload_state_grid(c, plane_ptr(c, swm::PL_U), plane_ptr(c, swm::PL_LO), u)) ||
+        (rc = upload_state_grid(c, plane_ptr(c, swm::PL_V), plane_ptr(c, swm::PL_LO + 1), v)) ||
+        (rc = upload_state_grid(c, plane_ptr(c, swm::PL_E), plane_ptr(c, swm::PL_LO + 2), eta)))
         return rc;
     const size_t pbytes = (size_t)c->plane * c->esz;
     for (int l = 0; l < 3; ++l) {
@@ -1244,7 +1303,7 @@ int swm_gpu_add_tracer(swm_ctx* c, const swm_tracer_desc* t, int32_t* tracer_id)
     if ((rc = alloc_state(c, nt + 1))) return rc;
     const int base = swm::PL_TRC + swm::PL_PER_TRC * nt;
     // the tracer's planes follow the context's current ping-pong / ring roles
-    if ((rc = upload_grid(c, plane_ptr(c, base + c->cur), t->CH))) return rc;
+    if ((rc = upload_state_grid(c, plane_ptr(c, base + c->cur), plane_ptr(c, base + swm::PL_TLO), t->CH))) return rc;
     if ((rc = upload_grid(c, plane_ptr(c, base + 5), t->C))) return rc;
     for (int l = 0; l < 3; ++l) {
         void* dst = plane_ptr(c, base + 2 + (c->ring + l) % 3);
@@ -1259,8 +1318,9 @@ int swm_gpu_add_tracer(swm_ctx* c, const swm_tracer_desc* t, int32_t* tracer_id)
     tr.kappa = t->kappa_h;
     tr.cbuf_valid = true;
     const int8_t* me = c->hmask[3].data();
+    std::vector<double> kap;
     if ((rc = upload_field(c, &tr.C0, t->C0, 0.0, me)) || (rc = upload_field(c, &tr.gC, t->gamma_C, 0.0, me)) ||
-        (rc = upload_field(c, &tr.impl, t->impl, 1.0, me)))
+        (rc = upload_field(c, &tr.impl, damp_static(c, t->impl, kap), c->fp32 ? 0.0 : 1.0, me)))
         return rc;
     c->trc.push_back(tr);
     c->march_slots = 0;   // the kernel instance (and its occupancy) depends on the tracer count
@@ -1394,10 +1454,11 @@ int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_o
     const int n0 = c->cur;
     const int r1 = (c->ring + 1) % 3, r0 = c->ring;
     const void* src = nullptr;
+    const void* lo_src = nullptr;   // low part of an FP32 state field
     if (tracer_id >= 0) {
         if (tracer_id >= (int)c->trc.size()) return fail(c, "no such tracer", 1);
         const int base = swm::PL_TRC + swm::PL_PER_TRC * tracer_id;
-        if (s == "CH") src = plane_ptr(c, base + n0);
+        if (s == "CH") src = plane_ptr(c, base + n0), lo_src = plane_ptr(c, base + swm::PL_TLO);
         else if (s == "G_CH") src = plane_ptr(c, base + 2 + r1);   // newest tendency (G(3) == G(2) after the shift)
         else if (s == "G_CH_M1") src = plane_ptr(c, base + 2 + r0);
         else if (s == "C" || s == "UHC" || s == "VHC") {
@@ -1430,9 +1491,13 @@ int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_o
         }
         HIPCHK(c, hipGetLastError());
         src = c->scratch;
-    } else if (s == "SWM_U" || s == "U" || s == "U_ND") src = plane_ptr(c, swm::PL_U + n0);   // u_nd = u without a solver
-    else if (s == "SWM_V" || s == "V" || s == "V_ND") src = plane_ptr(c, swm::PL_V + n0);
-    else if (s == "SWM_ETA" || s == "ETA") src = plane_ptr(c, swm::PL_E + n0);
+    } else if (s == "SWM_U" || s == "U" || s == "U_ND") {   // u_nd = u without a solver
+        src = plane_ptr(c, swm::PL_U + n0), lo_src = plane_ptr(c, swm::PL_LO);
+    } else if (s == "SWM_V" || s == "V" || s == "V_ND") {
+        src = plane_ptr(c, swm::PL_V + n0), lo_src = plane_ptr(c, swm::PL_LO + 1);
+    } else if (s == "SWM_ETA" || s == "ETA") {
+        src = plane_ptr(c, swm::PL_E + n0), lo_src = plane_ptr(c, swm::PL_LO + 2);
+    }
     else if (s == "G_U") src = plane_ptr(c, swm::PL_GU + r1);
     else if (s == "G_V") src = plane_ptr(c, swm::PL_GV + r1);
     else if (s == "G_ETA") src = plane_ptr(c, swm::PL_GE + r1);
@@ -1463,7 +1528,7 @@ int swm_gpu_sync(swm_ctx* c, const char* name, int32_t tracer_id, double* host_o
         src = c->diag[k];
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    return download_grid(c, host_out, src);
+    return download_state_grid(c, host_out, src, lo_src);
 }
 
 }  // extern "C"

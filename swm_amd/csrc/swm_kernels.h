@@ -137,8 +137,61 @@ __global__ void __launch_bounds__(256) pack_rowtab(const Statics<T>* __restrict_
 }
 
 // State planes in one allocation: u, v, eta at two levels, three AB slots per
-// field, then per tracer CH (two levels), G_CH (three slots) and the C buffer.
-enum : int { PL_U = 0, PL_V = 2, PL_E = 4, PL_GU = 6, PL_GV = 9, PL_GE = 12, PL_TRC = 15, PL_PER_TRC = 6 };
+// field, the low parts of u, v, eta (FP32 state), then per tracer CH (two
+// levels), G_CH (three slots), the C buffer and the low part of CH.
+enum : int {
+    PL_U = 0, PL_V = 2, PL_E = 4, PL_GU = 6, PL_GV = 9, PL_GE = 12, PL_LO = 15, PL_TRC = 18, PL_PER_TRC = 7,
+    PL_TLO = 6   // tracer plane of the low part of CH
+};
+
+// FP32 state with a carried low part.  A cell's value is hi + lo: hi in the
+// float plane the step reads, lo rounded to bfloat16 in a plane of its own
+// (|lo| <= ulp(hi)/2, so the pair holds about 32 significant bits).  The
+// step adds its increment to the pair with an exact two-sum and renormalises,
+// so the rounding of hi is carried to the next step instead of lost; without
+// it the rounding of the many small AB increments into hi accumulates to
+// O(steps * 2^-24) relative error in long runs.  The stored lo loses at most
+// 2^-33 |hi| per step.
+__device__ __forceinline__ float lo_dec(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+__device__ __forceinline__ unsigned short lo_enc(float x) {   // round to nearest even
+    const unsigned b = __float_as_uint(x);
+    return (unsigned short)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+struct Sum2f {
+    float s, e;
+};
+__device__ __forceinline__ Sum2f two_sum(float a, float b) {   // s + e == a + b exactly
+    const float s = a + b;
+    const float bp = s - a;
+    return {s, (a - (s - bp)) + (b - bp)};
+}
+// (hi + lo_in + inc) / impl with kap = 1 - 1/impl (0 where there is no
+// damping): the new (hi, lo).  The FP32 statics hold kap, not impl: impl =
+// 1 + dt*gamma is 1 + O(1e-5) for a linear drag, and a float keeps only a
+// few bits of its distance from 1 (ulp(1) = 1.2e-7), which changes the drag
+// by up to 1 % and, through the flow it damps, the amplitude of every field
+// by a relative O(1e-8) per step.  x * (1 - kap) is formed as x - x*kap with
+// the product's FMA error and the difference's two-sum error carried in lo.
+__device__ __forceinline__ float carry_step(float hi, unsigned short lo_in, float inc, float kap,
+                                            unsigned short& lo_out) {
+    const Sum2f t = two_sum(hi, inc + lo_dec(lo_in));
+    float q = t.s, e = t.e;
+    if (kap != 0.0f) {
+        const float p = t.s * kap;
+        const float ep = __builtin_fmaf(t.s, kap, -p);   // t.s * kap == p + ep
+        const Sum2f w = two_sum(t.s, -p);
+        q = w.s;
+        e = (w.e - ep) + (e - e * kap);
+    }
+    const float h = q + e;
+    const bool fin = __builtin_isfinite(q);   // Inf / NaN: the plain float result, no low part
+    lo_out = fin ? lo_enc(e - (h - q)) : (unsigned short)0;
+    return fin ? h : q;
+}
+template <typename T>
+__device__ __forceinline__ const unsigned short* lo_plane(const T* st, long long pl, int k) {
+    return (const unsigned short*)(st + (long long)k * pl);
+}
 
 // One forcing stream's place on its time axis for a step: records o0 and o1
 // (element offsets) bracket the model time; the value is rec[o0] when exact,
@@ -396,6 +449,7 @@ struct TileRegs {
     T g[NO][6];
     T gt[NO][NTR][2];
     T eo[NO];    // eta at the outputs
+    unsigned short lo[NO][3 + NTR];   // low parts of u, v, eta, CH at the outputs (FP32 state)
 };
 
 template <int TX, int TY>
@@ -445,6 +499,13 @@ __device__ __forceinline__ void tile_load_hist(const Dyn<T>& d, const int t, RG&
                     const T* tb = st + (PL_TRC + PL_PER_TRC * n + 2) * pl;
                     pt[q][n][0] = gld(tb, r0 * pl + off);
                     pt[q][n][1] = gld(tb, r1 * pl + off);
+                }
+                if constexpr (sizeof(T) == 4) {
+#pragma unroll
+                    for (int f = 0; f < 3; ++f) R.lo[q][f] = gld(lo_plane(st, pl, PL_LO + f), off);
+#pragma unroll
+                    for (int n = 0; n < NTRC; ++n)
+                        R.lo[q][3 + n] = gld(lo_plane(st, pl, PL_TRC + PL_PER_TRC * n + PL_TLO), off);
                 }
             }
         }
@@ -674,9 +735,9 @@ __device__ __forceinline__ void tile_stage(const Dyn<T>& d, const int t, const R
 // Phases 2 and 3 of tile t, after its LDS stage and a barrier: derived fields,
 // tendencies, AB integration and stores.  pg/pt: the outputs' AB history.
 template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, bool DYN = true, typename L, typename G,
-          typename GT, typename GE>
+          typename GT, typename GE, typename LO>
 __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& lds, const G& pg, const GT& pt,
-                                             const GE& pe) {
+                                             const GE& pe, const LO& plo) {
     constexpr int NT = 256;
     constexpr int EW = TX + 2;
     constexpr int EH = TY + 2;
@@ -1005,30 +1066,56 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
         const T Gu3 = ou ? Gu : pg[q][1];
         const T Gv3 = ov ? Gv : pg[q][3];
         const T Ge3 = oe ? Ge : pg[q][5];
-        T un = s_u[k], vn = s_v[k], en = pe[q];
-        un = madd(un, d.dtc1, pg[q][0]);
-        vn = madd(vn, d.dtc1, pg[q][2]);
-        en = madd(en, d.dtc1, pg[q][4]);
-        if (d.tstep >= 2) {
-            un = madd(un, d.dtc2, pg[q][1]);
-            vn = madd(vn, d.dtc2, pg[q][3]);
-            en = madd(en, d.dtc2, pg[q][5]);
+        // the damping static at (i, j): FP32 contexts hold 1 - 1/impl there
+        auto fdv = [&](auto& f, int e, bool ocean) -> T { return f.full ? gld(f.full, off) : rvs(ocean, e, e + 1); };
+        T un, vn, en;
+        unsigned short lu = 0, lv = 0, le = 0;
+        if constexpr (sizeof(T) == 4) {   // increments added to the carried pair
+            T iu = d.dtc1 * pg[q][0], iv = d.dtc1 * pg[q][2], ie = d.dtc1 * pg[q][4];
+            if (d.tstep >= 2) {
+                iu = madd(iu, d.dtc2, pg[q][1]);
+                iv = madd(iv, d.dtc2, pg[q][3]);
+                ie = madd(ie, d.dtc2, pg[q][5]);
+            }
+            if (d.tstep >= 3) {
+                iu = madd(iu, d.dtc3, Gu3);
+                iv = madd(iv, d.dtc3, Gv3);
+                ie = madd(ie, d.dtc3, Ge3);
+            }
+            un = carry_step(s_u[k], plo[q][0], iu, fdv(a.imu, RT_IMU, ou), lu);
+            vn = carry_step(s_v[k], plo[q][1], iv, fdv(a.imv, RT_IMV, ov), lv);
+            en = carry_step(pe[q], plo[q][2], ie, fdv(a.ime, RT_IME, oe), le);
+        } else {
+            un = s_u[k], vn = s_v[k], en = pe[q];
+            un = madd(un, d.dtc1, pg[q][0]);
+            vn = madd(vn, d.dtc1, pg[q][2]);
+            en = madd(en, d.dtc1, pg[q][4]);
+            if (d.tstep >= 2) {
+                un = madd(un, d.dtc2, pg[q][1]);
+                vn = madd(vn, d.dtc2, pg[q][3]);
+                en = madd(en, d.dtc2, pg[q][5]);
+            }
+            if (d.tstep >= 3) {
+                un = madd(un, d.dtc3, Gu3);
+                vn = madd(vn, d.dtc3, Gv3);
+                en = madd(en, d.dtc3, Ge3);
+            }
+            un = flq(un, a.imu, RT_IMU, RT_RIMU, off, ou);
+            vn = flq(vn, a.imv, RT_IMV, RT_RIMV, off, ov);
+            en = flq(en, a.ime, RT_IME, RT_RIME, off, oe);
         }
-        if (d.tstep >= 3) {
-            un = madd(un, d.dtc3, Gu3);
-            vn = madd(vn, d.dtc3, Gv3);
-            en = madd(en, d.dtc3, Ge3);
-        }
-        un = flq(un, a.imu, RT_IMU, RT_RIMU, off, ou);
-        vn = flq(vn, a.imv, RT_IMV, RT_RIMV, off, ov);
-        en = flq(en, a.ime, RT_IME, RT_RIME, off, oe);
         if (FN) {   // SWM_timestep_step: where (eta < minD - H) eta = minD - H
             const T lim = a.minD - fl(a.He, RT_HE, off, oe);
-            if (en < lim) en = lim;
+            if (en < lim) { en = lim; le = 0; }
         }
         gst(st, (PL_U + 1 - d.cur) * pl + off, un);
         gst(st, (PL_V + 1 - d.cur) * pl + off, vn);
         gst(st, (PL_E + 1 - d.cur) * pl + off, en);
+        if constexpr (sizeof(T) == 4) {
+            gst((unsigned short*)lo_plane(st, pl, PL_LO), off, lu);
+            gst((unsigned short*)lo_plane(st, pl, PL_LO + 1), off, lv);
+            gst((unsigned short*)lo_plane(st, pl, PL_LO + 2), off, le);
+        }
         gst(st, (PL_GU + r2) * pl + off, Gu3);
         gst(st, (PL_GV + r2) * pl + off, Gv3);
         gst(st, (PL_GE + r2) * pl + off, Ge3);
@@ -1087,10 +1174,20 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             g = g + diff;
             g = g - (fl(a.gC[n], RT_TRC + RT_PER_TRC * n + 2, off, oe) * s_D[k]) * (C - fl(a.C0[n], RT_TRC + RT_PER_TRC * n + 4, off, oe));
             g = g + forcing;
-            T chn = CH + d.dtc1 * pt[q][n][0];
-            if (d.tstep >= 2) chn = madd(chn, d.dtc2, pt[q][n][1]);
-            if (d.tstep >= 3) chn = madd(chn, d.dtc3, g);
-            chn = flq(chn, a.timpl[n], RT_TRC + RT_PER_TRC * n, RT_TRC + RT_PER_TRC * n + 6, off, oe);
+            T chn;
+            if constexpr (sizeof(T) == 4) {
+                T inc = d.dtc1 * pt[q][n][0];
+                if (d.tstep >= 2) inc = madd(inc, d.dtc2, pt[q][n][1]);
+                if (d.tstep >= 3) inc = madd(inc, d.dtc3, g);
+                unsigned short lc;
+                chn = carry_step(CH, plo[q][3 + n], inc, fdv(a.timpl[n], RT_TRC + RT_PER_TRC * n, oe), lc);
+                gst((unsigned short*)lo_plane(tb, pl, PL_TLO), off, lc);
+            } else {
+                chn = CH + d.dtc1 * pt[q][n][0];
+                if (d.tstep >= 2) chn = madd(chn, d.dtc2, pt[q][n][1]);
+                if (d.tstep >= 3) chn = madd(chn, d.dtc3, g);
+                chn = flq(chn, a.timpl[n], RT_TRC + RT_PER_TRC * n, RT_TRC + RT_PER_TRC * n + 6, off, oe);
+            }
             gst(tb, (1 - d.cur) * pl + off, chn);
             gst(tb, (2 + r2) * pl + off, g);
         }
@@ -1116,7 +1213,7 @@ __device__ __forceinline__ void tile_body(const Dyn<T>& d, const int t, L& lds) 
     tile_stage<T, VAR, NTRC, TX, TY, DIAG>(d, t, R, lds);
     if (LATE) tile_load_hist<T, VAR, NTRC, TX, TY, DIAG>(d, t, R);
     __syncthreads();
-    tile_compute<T, VAR, NTRC, TX, TY, DIAG, DYN>(d, t, lds, R.g, R.gt, R.eo);
+    tile_compute<T, VAR, NTRC, TX, TY, DIAG, DYN>(d, t, lds, R.g, R.gt, R.eo, R.lo);
 }
 
 template <typename T, int VAR, int NTRC, int TX, int TY, bool DIAG, bool LATE = false>
@@ -1151,9 +1248,12 @@ step_kernel_p(const Dyn<T> d) {
         if (t > t0) __syncthreads();   // phase 3 of the previous tile is done with LDS
         tile_stage<T, VAR, NTRC, TX, TY, false>(d, t, R, lds);
         T g[S::NO][6], gt[S::NO][NTR][2], eo[S::NO];
+        unsigned short lo[S::NO][3 + NTR];
 #pragma unroll
         for (int q = 0; q < S::NO; ++q) {
             eo[q] = R.eo[q];
+#pragma unroll
+            for (int k = 0; k < 3 + NTR; ++k) lo[q][k] = R.lo[q][k];
 #pragma unroll
             for (int k = 0; k < 6; ++k) g[q][k] = R.g[q][k];
 #pragma unroll
@@ -1161,7 +1261,7 @@ step_kernel_p(const Dyn<T> d) {
         }
         __syncthreads();
         if (t + 1 < t1) tile_load<T, VAR, NTRC, TX, TY, false>(d, t + 1, R);
-        tile_compute<T, VAR, NTRC, TX, TY, false>(d, t, lds, g, gt, eo);
+        tile_compute<T, VAR, NTRC, TX, TY, false>(d, t, lds, g, gt, eo, lo);
     }
 }
 
@@ -1333,7 +1433,7 @@ march_kernel(const Dyn<T> d) {
             r.Cb[n] = ((d.cbuf >> n) & 1) ? ldg(tb + 5 * pl, gi) : T(0);
         }
     };
-    auto load_hist = [&](int k, T* g, T (*gt)[2]) {
+    auto load_hist = [&](int k, T* g, T (*gt)[2], unsigned short* lo) {
         const T* const b = st + (size_t)k * P;
         g[0] = ldg(b + (PL_GU + r0) * pl, gi); g[1] = ldg(b + (PL_GU + r1) * pl, gi);
         g[2] = ldg(b + (PL_GV + r0) * pl, gi); g[3] = ldg(b + (PL_GV + r1) * pl, gi);
@@ -1343,6 +1443,13 @@ march_kernel(const Dyn<T> d) {
             const T* tb = b + (PL_TRC + PL_PER_TRC * n + 2) * pl;
             gt[n][0] = ldg(tb + r0 * pl, gi);
             gt[n][1] = ldg(tb + r1 * pl, gi);
+        }
+        if constexpr (sizeof(T) == 4) {   // low parts of the FP32 state (rows of lo planes at pitch P)
+            const size_t ro = (size_t)k * P;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) lo[f] = ldg(lo_plane(st, pl, PL_LO + f) + ro, gi);
+#pragma unroll
+            for (int n = 0; n < NTRC; ++n) lo[3 + n] = ldg(lo_plane(st, pl, PL_TRC + PL_PER_TRC * n + PL_TLO) + ro, gi);
         }
     };
 
@@ -1360,7 +1467,8 @@ march_kernel(const Dyn<T> d) {
         if (k < jend) load_raw(k + 1, N);
         // AB history of row k-1, the row this call writes
         T g[6], gt[NTR][2];
-        if (k > j0) load_hist(k - 1, g, gt);
+        unsigned short lo[3 + NTR];
+        if (k > j0) load_hist(k - 1, g, gt, lo);
 
         // ---- row k: fields local to the row
         K.D = FN ? (OE(K.m) ? (K.eta + He) : T(1)) : He;   // computeD; linear variants alias H_eta
@@ -1519,31 +1627,57 @@ march_kernel(const Dyn<T> d) {
                     const T Gu3 = ou ? Gu : g[1];
                     const T Gv3 = ov ? Gv : g[3];
                     const T Ge3 = oe ? Ge : g[5];
-                    T un = P1.u, vn = P1.v, en = P1.eta;
-                    un = madd(un, d.dtc1, g[0]);
-                    vn = madd(vn, d.dtc1, g[2]);
-                    en = madd(en, d.dtc1, g[4]);
-                    if (d.tstep >= 2) {
-                        un = madd(un, d.dtc2, g[1]);
-                        vn = madd(vn, d.dtc2, g[3]);
-                        en = madd(en, d.dtc2, g[5]);
+                    const T dvu = fld_r(a.imu, ro, gi, j, ou), dvv = fld_r(a.imv, ro, gi, j, ov),
+                            dve = fld_r(a.ime, ro, gi, j, oe);
+                    T un, vn, en;
+                    unsigned short lu = 0, lv = 0, le = 0;
+                    if constexpr (sizeof(T) == 4) {   // increments added to the carried pair
+                        T iu = d.dtc1 * g[0], iv = d.dtc1 * g[2], ie = d.dtc1 * g[4];
+                        if (d.tstep >= 2) {
+                            iu = madd(iu, d.dtc2, g[1]);
+                            iv = madd(iv, d.dtc2, g[3]);
+                            ie = madd(ie, d.dtc2, g[5]);
+                        }
+                        if (d.tstep >= 3) {
+                            iu = madd(iu, d.dtc3, Gu3);
+                            iv = madd(iv, d.dtc3, Gv3);
+                            ie = madd(ie, d.dtc3, Ge3);
+                        }
+                        un = carry_step(P1.u, lo[0], iu, dvu, lu);   // FP32: dvu etc. hold 1 - 1/impl
+                        vn = carry_step(P1.v, lo[1], iv, dvv, lv);
+                        en = carry_step(P1.eta, lo[2], ie, dve, le);
+                    } else {
+                        un = P1.u, vn = P1.v, en = P1.eta;
+                        un = madd(un, d.dtc1, g[0]);
+                        vn = madd(vn, d.dtc1, g[2]);
+                        en = madd(en, d.dtc1, g[4]);
+                        if (d.tstep >= 2) {
+                            un = madd(un, d.dtc2, g[1]);
+                            vn = madd(vn, d.dtc2, g[3]);
+                            en = madd(en, d.dtc2, g[5]);
+                        }
+                        if (d.tstep >= 3) {
+                            un = madd(un, d.dtc3, Gu3);
+                            vn = madd(vn, d.dtc3, Gv3);
+                            en = madd(en, d.dtc3, Ge3);
+                        }
+                        un = dv(un, dvu);
+                        vn = dv(vn, dvv);
+                        en = dv(en, dve);
                     }
-                    if (d.tstep >= 3) {
-                        un = madd(un, d.dtc3, Gu3);
-                        vn = madd(vn, d.dtc3, Gv3);
-                        en = madd(en, d.dtc3, Ge3);
-                    }
-                    un = dv(un, fld_r(a.imu, ro, gi, j, ou));
-                    vn = dv(vn, fld_r(a.imv, ro, gi, j, ov));
-                    en = dv(en, fld_r(a.ime, ro, gi, j, oe));
                     if (FN) {   // where (eta < minD - H) eta = minD - H
                         const T lim = a.minD - fld_r(a.He, ro, gi, j, oe);
-                        if (en < lim) en = lim;
+                        if (en < lim) { en = lim; le = 0; }
                     }
                     T* const wb = st + ro;
                     stg(wb + (PL_U + 1 - d.cur) * pl, gi, un);
                     stg(wb + (PL_V + 1 - d.cur) * pl, gi, vn);
                     stg(wb + (PL_E + 1 - d.cur) * pl, gi, en);
+                    if constexpr (sizeof(T) == 4) {
+                        stg((unsigned short*)lo_plane(st, pl, PL_LO) + ro, gi, lu);
+                        stg((unsigned short*)lo_plane(st, pl, PL_LO + 1) + ro, gi, lv);
+                        stg((unsigned short*)lo_plane(st, pl, PL_LO + 2) + ro, gi, le);
+                    }
                     stg(wb + (PL_GU + r2) * pl, gi, Gu3);
                     stg(wb + (PL_GV + r2) * pl, gi, Gv3);
                     stg(wb + (PL_GE + r2) * pl, gi, Ge3);
@@ -1589,10 +1723,21 @@ march_kernel(const Dyn<T> d) {
                         gg = gg + diff;
                         gg = gg - (fld_r(a.gC[n], ro, gi, j, oe) * P1.D) * (C - fld_r(a.C0[n], ro, gi, j, oe));
                         gg = gg + forcing;
-                        T chn = CH + d.dtc1 * gt[n][0];
-                        if (d.tstep >= 2) chn = madd(chn, d.dtc2, gt[n][1]);
-                        if (d.tstep >= 3) chn = madd(chn, d.dtc3, gg);
-                        chn = dv(chn, fld_r(a.timpl[n], ro, gi, j, oe));
+                        const T dvc = fld_r(a.timpl[n], ro, gi, j, oe);
+                        T chn;
+                        if constexpr (sizeof(T) == 4) {
+                            T inc = d.dtc1 * gt[n][0];
+                            if (d.tstep >= 2) inc = madd(inc, d.dtc2, gt[n][1]);
+                            if (d.tstep >= 3) inc = madd(inc, d.dtc3, gg);
+                            unsigned short lc;
+                            chn = carry_step(CH, lo[3 + n], inc, dvc, lc);
+                            stg((unsigned short*)lo_plane(st, pl, PL_TRC + PL_PER_TRC * n + PL_TLO) + ro, gi, lc);
+                        } else {
+                            chn = CH + d.dtc1 * gt[n][0];
+                            if (d.tstep >= 2) chn = madd(chn, d.dtc2, gt[n][1]);
+                            if (d.tstep >= 3) chn = madd(chn, d.dtc3, gg);
+                            chn = dv(chn, dvc);
+                        }
                         stg(tb + (1 - d.cur) * pl, gi, chn);
                         stg(tb + (2 + r2) * pl, gi, gg);
                     }

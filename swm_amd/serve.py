@@ -90,14 +90,15 @@ def build_setup(r: Request):
 
 def footprint_bytes(Nx: int, Ny: int, ntracers: int, precision: int) -> int:
     """Device memory of one context: the state planes (u, v, eta x2, 9 AB slots,
-    6 per tracer), one scratch plane, the mask and the per-field static
-    arrays (an upper bound: every static field stored in full)."""
+    3 low-part planes, 7 per tracer), one scratch plane, the mask and the
+    per-field static arrays (an upper bound: every static field stored in
+    full)."""
     esz = 4 if precision == gpu.FP32 else 8
     align = 128 // esz
     P = (Nx + align - 1) // align * align
     plane = P * Ny * esz
     statics = 19 + 3 * ntracers
-    return plane * (15 + 6 * ntracers + 1 + statics) + P * Ny + (1 << 20)
+    return plane * (18 + 7 * ntracers + 1 + statics) + P * Ny + (1 << 20)
 
 
 class Server:

@@ -1,0 +1,71 @@
+"""The FP32 storage path's error contract over the lengths the ensemble and
+serving workloads run: relative max-norm eta error <= 1e-4 against the FP64
+path (itself bit-identical to the oracle) after 5000 and 20000 steps.
+
+Two things keep the FP32 error from growing with the number of steps:
+* the implicit damping is held as kap = 1 - 1/impl and applied as x - x*kap
+  (a linear drag gives impl = 1 + O(1e-5), which a float holds to only a few
+  bits, so the drag - and with it the amplitude of the damped flow - would
+  drift by a relative O(1e-8) per step);
+* the state carries a bfloat16 low part, so the rounding of the many small
+  AB increments is not lost.
+"""
+import numpy as np
+import pytest
+
+from swm_amd import configs, gpu, host
+
+pytestmark = pytest.mark.gpu
+
+
+def eta_err(a, b, s):
+    oc = s.grids["eta"].ocean == 1
+    e64, e32 = a.get("SWM_ETA")[oc], b.get("SWM_ETA")[oc]
+    return float(np.abs(e32 - e64).max() / np.abs(e64).max())
+
+
+# ensemble members with the largest FP32 drift before the damping fix
+# (deep and shallow, dt from 5 to 21 s, 128 to 768 cells a side)
+MEMBERS = [59, 37, 30, 39]
+
+
+@pytest.mark.parametrize("k", MEMBERS)
+def test_fp32_eta_error_over_twenty_thousand_steps(gpu_lib, k):
+    p = configs.ensemble_params(0, 64)[k]
+    s = configs.ensemble_member(nsteps=20000, **p)
+    a = gpu.SwmGpu(s, gpu.FP64).init_from_setup()
+    b = gpu.SwmGpu(s, gpu.FP32).init_from_setup()
+    for n in (5000, 20000):
+        a.step(n - a.itt)
+        b.step(n - b.itt)
+        err = eta_err(a, b, s)
+        print(f"member {k} ({s.Nx}x{s.Ny}, H={p['H0']:.0f}, dt={s.dt}) step {n}: eta error {err:.3g}")
+        assert err <= 1e-4, (k, n, err)
+    a.close()
+    b.close()
+
+
+def test_fp32_linear_drag_decay_closed_form(gpu_lib):
+    """A uniform zonal flow in a flat, non-rotating, periodic channel with a
+    linear drag and nothing else decays as u0 / impl**n (the implicit step is
+    u_new = u / impl).  impl = 1 + dt*r = 1 + 6e-6 is not a float; FP32 must
+    still follow the closed form to float precision, not to the 1e-3 relative
+    drag error a float impl would give."""
+    dt, r, U = 6.0, 1e-6, 0.25
+    s = configs.basin(32, 16, 5e3, 5e3, variant=host.FULLY_NONLINEAR, H0=1000.0, dt=dt, drag=r,
+                      side_walls=False)
+    s.u0 = np.full(s.shape, U)
+    s.v0 = np.zeros(s.shape)
+    s.eta0 = np.zeros(s.shape)
+    s = host.finalize(s)
+    impl = s.impl_u[s.grids["u"].ocean == 1]
+    assert np.all(impl == impl[0]) and impl[0] != np.float32(impl[0])
+    ok = s.grids["u"].ocean == 1
+    for prec, bar in ((gpu.FP64, 1e-13), (gpu.FP32, 1e-5)):
+        g = gpu.SwmGpu(s, prec).init_from_setup()
+        for n in (1000, 20000):
+            g.step(n - g.itt)
+            want = U / impl[0] ** n
+            u = g.get("SWM_U")[ok]
+            assert np.abs(u - want).max() <= bar * want, (prec, n, np.abs(u - want).max() / want)
+        g.close()
