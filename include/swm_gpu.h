@@ -42,7 +42,10 @@ enum swm_flags {
     SWM_QUADRATIC_BOTTOM_FRICTION = 2,      /* QUADRATIC_BOTTOM_FRICTION */
     SWM_BAROTROPIC = 4,                     /* BAROTROPIC */
     SWM_TAU_SCALE = 8,                      /* TAU_SCALE defined (value in swm_desc.tau_scale) */
-    SWM_ELLIPTIC_SOLVER = 16                /* built with CALC_LIB_ELLIPTIC_SOLVER = ElSolv_SOR (include/calc_lib.h:7-18) */
+    SWM_ELLIPTIC_SOLVER = 16,               /* built with CALC_LIB_ELLIPTIC_SOLVER = ElSolv_SOR (include/calc_lib.h:7-18) */
+    SWM_FP32_LOW_PART = 32                  /* no model.h counterpart: an SWM_FP32 context carries a bfloat16
+                                               low part of u, v, eta and CH (32 significant bits of state,
+                                               4 B more per field and step); ignored in FP64 */
 };
 enum swm_precision { SWM_FP64 = 0, SWM_FP32 = 1 };
 enum swm_grid_id { SWM_GRID_H = 0, SWM_GRID_U = 1, SWM_GRID_V = 2, SWM_GRID_ETA = 3 };

@@ -461,10 +461,13 @@ double lo_dec_host(uint16_t h) {
     return f;
 }
 
-// A state field: FP64 as is; FP32 as the pair hi = (float)x in the field's
-// plane and lo = x - hi (bfloat16) in its low-part plane.
+bool carries_low_part(const swm_ctx* c) { return c->fp32 && (c->d.flags & SWM_FP32_LOW_PART); }
+
+// A state field: FP64 as is; FP32 as hi = (float)x in the field's plane and,
+// with SWM_FP32_LOW_PART, lo = x - hi (bfloat16) in its low-part plane (the
+// low-part planes stay zero without it).
 int upload_state_grid(swm_ctx* c, void* dst, void* lo_dst, const double* src) {
-    if (!c->fp32) return upload_grid(c, dst, src);
+    if (!carries_low_part(c)) return upload_grid(c, dst, src);
     const size_t n = (size_t)c->Nx * c->Ny;
     std::vector<float> f(n);
     std::vector<uint16_t> l(n);
@@ -478,7 +481,7 @@ int upload_state_grid(swm_ctx* c, void* dst, void* lo_dst, const double* src) {
 }
 
 int download_state_grid(swm_ctx* c, double* dst, const void* src, const void* lo_src) {
-    if (!c->fp32 || !lo_src) return download_grid(c, dst, src);
+    if (!carries_low_part(c) || !lo_src) return download_grid(c, dst, src);
     const size_t n = (size_t)c->Nx * c->Ny;
     std::vector<float> f(n);
     std::vector<uint16_t> l(n);
@@ -657,6 +660,7 @@ int write_statics_t(swm_ctx* c) {
     Statics<T> a{};
     a.Nx = c->Nx; a.Ny = c->Ny; a.P = c->P;
     a.mix = (d.flags & SWM_LATERAL_MIXING) ? 1 : 0;
+    a.carry = (c->fp32 && (d.flags & SWM_FP32_LOW_PART)) ? 1 : 0;
     a.quad = (d.flags & SWM_QUADRATIC_BOTTOM_FRICTION) ? 1 : 0;
     a.barotropic = (d.flags & SWM_BAROTROPIC) ? 1 : 0;
     a.has_tau_scale = (d.flags & SWM_TAU_SCALE) ? 1 : 0;

@@ -58,6 +58,7 @@ template <typename T>
 struct Statics {
     int Nx, Ny, P;
     int mix, quad, barotropic, has_tau_scale;
+    int carry;           // FP32 state carries its bfloat16 low part (lo planes read and written)
     int has_taux, has_tauy, has_ox, has_oy, has_oe;
     T G, A, dL, dT, RHO0, minD, lbc, tau_scale, AdT, eps;
     T rdL, rdT, rAdT;    // 1/dL, 1/dT, 1/AdT (correctly rounded, for dvr)
@@ -501,11 +502,16 @@ __device__ __forceinline__ void tile_load_hist(const Dyn<T>& d, const int t, RG&
                     pt[q][n][1] = gld(tb, r1 * pl + off);
                 }
                 if constexpr (sizeof(T) == 4) {
+                    if (a.carry) {
 #pragma unroll
-                    for (int f = 0; f < 3; ++f) R.lo[q][f] = gld(lo_plane(st, pl, PL_LO + f), off);
+                        for (int f = 0; f < 3; ++f) R.lo[q][f] = gld(lo_plane(st, pl, PL_LO + f), off);
 #pragma unroll
-                    for (int n = 0; n < NTRC; ++n)
-                        R.lo[q][3 + n] = gld(lo_plane(st, pl, PL_TRC + PL_PER_TRC * n + PL_TLO), off);
+                        for (int n = 0; n < NTRC; ++n)
+                            R.lo[q][3 + n] = gld(lo_plane(st, pl, PL_TRC + PL_PER_TRC * n + PL_TLO), off);
+                    } else {
+#pragma unroll
+                        for (int f = 0; f < 3 + NTRC; ++f) R.lo[q][f] = 0;
+                    }
                 }
             }
         }
@@ -1112,9 +1118,11 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
         gst(st, (PL_V + 1 - d.cur) * pl + off, vn);
         gst(st, (PL_E + 1 - d.cur) * pl + off, en);
         if constexpr (sizeof(T) == 4) {
-            gst((unsigned short*)lo_plane(st, pl, PL_LO), off, lu);
-            gst((unsigned short*)lo_plane(st, pl, PL_LO + 1), off, lv);
-            gst((unsigned short*)lo_plane(st, pl, PL_LO + 2), off, le);
+            if (a.carry) {
+                gst((unsigned short*)lo_plane(st, pl, PL_LO), off, lu);
+                gst((unsigned short*)lo_plane(st, pl, PL_LO + 1), off, lv);
+                gst((unsigned short*)lo_plane(st, pl, PL_LO + 2), off, le);
+            }
         }
         gst(st, (PL_GU + r2) * pl + off, Gu3);
         gst(st, (PL_GV + r2) * pl + off, Gv3);
@@ -1181,7 +1189,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                 if (d.tstep >= 3) inc = madd(inc, d.dtc3, g);
                 unsigned short lc;
                 chn = carry_step(CH, plo[q][3 + n], inc, fdv(a.timpl[n], RT_TRC + RT_PER_TRC * n, oe), lc);
-                gst((unsigned short*)lo_plane(tb, pl, PL_TLO), off, lc);
+                if (a.carry) gst((unsigned short*)lo_plane(tb, pl, PL_TLO), off, lc);
             } else {
                 chn = CH + d.dtc1 * pt[q][n][0];
                 if (d.tstep >= 2) chn = madd(chn, d.dtc2, pt[q][n][1]);
@@ -1447,9 +1455,14 @@ march_kernel(const Dyn<T> d) {
         if constexpr (sizeof(T) == 4) {   // low parts of the FP32 state (rows of lo planes at pitch P)
             const size_t ro = (size_t)k * P;
 #pragma unroll
-            for (int f = 0; f < 3; ++f) lo[f] = ldg(lo_plane(st, pl, PL_LO + f) + ro, gi);
+            for (int f = 0; f < 3 + NTRC; ++f) lo[f] = 0;
+            if (a.carry) {
 #pragma unroll
-            for (int n = 0; n < NTRC; ++n) lo[3 + n] = ldg(lo_plane(st, pl, PL_TRC + PL_PER_TRC * n + PL_TLO) + ro, gi);
+                for (int f = 0; f < 3; ++f) lo[f] = ldg(lo_plane(st, pl, PL_LO + f) + ro, gi);
+#pragma unroll
+                for (int n = 0; n < NTRC; ++n)
+                    lo[3 + n] = ldg(lo_plane(st, pl, PL_TRC + PL_PER_TRC * n + PL_TLO) + ro, gi);
+            }
         }
     };
 
@@ -1674,9 +1687,11 @@ march_kernel(const Dyn<T> d) {
                     stg(wb + (PL_V + 1 - d.cur) * pl, gi, vn);
                     stg(wb + (PL_E + 1 - d.cur) * pl, gi, en);
                     if constexpr (sizeof(T) == 4) {
-                        stg((unsigned short*)lo_plane(st, pl, PL_LO) + ro, gi, lu);
-                        stg((unsigned short*)lo_plane(st, pl, PL_LO + 1) + ro, gi, lv);
-                        stg((unsigned short*)lo_plane(st, pl, PL_LO + 2) + ro, gi, le);
+                        if (a.carry) {
+                            stg((unsigned short*)lo_plane(st, pl, PL_LO) + ro, gi, lu);
+                            stg((unsigned short*)lo_plane(st, pl, PL_LO + 1) + ro, gi, lv);
+                            stg((unsigned short*)lo_plane(st, pl, PL_LO + 2) + ro, gi, le);
+                        }
                     }
                     stg(wb + (PL_GU + r2) * pl, gi, Gu3);
                     stg(wb + (PL_GV + r2) * pl, gi, Gv3);
@@ -1731,7 +1746,8 @@ march_kernel(const Dyn<T> d) {
                             if (d.tstep >= 3) inc = madd(inc, d.dtc3, gg);
                             unsigned short lc;
                             chn = carry_step(CH, lo[3 + n], inc, dvc, lc);
-                            stg((unsigned short*)lo_plane(st, pl, PL_TRC + PL_PER_TRC * n + PL_TLO) + ro, gi, lc);
+                            if (a.carry)
+                                stg((unsigned short*)lo_plane(st, pl, PL_TRC + PL_PER_TRC * n + PL_TLO) + ro, gi, lc);
                         } else {
                             chn = CH + d.dtc1 * gt[n][0];
                             if (d.tstep >= 2) chn = madd(chn, d.dtc2, gt[n][1]);

@@ -15,7 +15,8 @@ module swm_gpu_shim
   integer(c_int32_t), parameter, public :: SWM_FULLY_NONLINEAR = 1, SWM_LINEARISED_MEAN_STATE = 2, &
                                            SWM_LINEARISED_STATE_OF_REST = 3
   integer(c_int32_t), parameter, public :: SWM_LATERAL_MIXING = 1, SWM_QUADRATIC_BOTTOM_FRICTION = 2, &
-                                           SWM_BAROTROPIC = 4, SWM_TAU_SCALE = 8, SWM_ELLIPTIC_SOLVER = 16
+                                           SWM_BAROTROPIC = 4, SWM_TAU_SCALE = 8, SWM_ELLIPTIC_SOLVER = 16, &
+                                           SWM_FP32_LOW_PART = 32
   integer(c_int32_t), parameter, public :: SWM_FP64 = 0, SWM_FP32 = 1
   integer(c_int32_t), parameter, public :: SWM_COMP_X = 0, SWM_COMP_Y = 1, SWM_COMP_ETA = 2
   integer(c_int32_t), parameter, public :: SWM_STREAM_WIND = 0, SWM_STREAM_CUSTOM = 1, SWM_STREAM_HEATING = 2
@@ -79,6 +80,15 @@ module swm_gpu_shim
       real(c_double), value :: rec_dt
       real(c_double), intent(in) :: records(*)
       integer(c_int32_t), intent(out) :: id
+    end function
+    ! DFF_initDynFromFile (src/dynFromFile_module.f90:56-95): one dataset of u, v or eta
+    integer(c_int) function swm_gpu_add_dyn_stream(ctx, field, nrec, rec_dt, chunk_size, records) &
+        bind(C, name="swm_gpu_add_dyn_stream")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: field, nrec, chunk_size
+      real(c_double), value :: rec_dt
+      real(c_double), intent(in) :: records(*)
     end function
     integer(c_int) function swm_gpu_step(ctx, nsteps) bind(C, name="swm_gpu_step")
       import :: c_int, c_ptr, c_int64_t
@@ -163,6 +173,7 @@ module swm_gpu_shim
   end interface
 
   public :: swm_gpu_create, swm_gpu_upload_state, swm_gpu_add_tracer, swm_gpu_add_forcing_stream, &
+            swm_gpu_add_dyn_stream, &
             swm_gpu_step, swm_gpu_itt, &
             swm_gpu_sync, swm_gpu_sor, swm_gpu_velocity_potential, swm_gpu_avg_add, swm_gpu_avg_read, &
             swm_gpu_sums, swm_gpu_sums_noalloc, swm_gpu_elliptic_stats, swm_gpu_tune, swm_gpu_last_error, &

@@ -25,6 +25,7 @@ ABI_VERSION = 2
 
 FULLY_NONLINEAR, LINEARISED_MEAN_STATE, LINEARISED_STATE_OF_REST = 1, 2, 3
 LATERAL_MIXING, QUADRATIC_BOTTOM_FRICTION, BAROTROPIC, TAU_SCALE, ELLIPTIC_SOLVER = 1, 2, 4, 8, 16
+FP32_LOW_PART = 32              # SWM_FP32_LOW_PART
 FP64, FP32 = 0, 1
 
 EXPORTED = [
@@ -204,14 +205,17 @@ def initial_D(s, eta):
 class SwmGpu:
     """Device-resident SWM model (C-ABI context) built from a host Setup."""
 
-    def __init__(self, s, precision: int = FP64):
+    def __init__(self, s, precision: int = FP64, low_part: bool = False):
+        """low_part: an FP32 context carries a bfloat16 low part of its state
+        (SWM_FP32_LOW_PART)."""
         self.L = load_library()
         self.s = s
         self.precision = precision
         keep = self._keep = []
         d = Desc()
         d.abi_version = ABI_VERSION
-        d.Nx, d.Ny, d.variant, d.flags, d.precision = s.Nx, s.Ny, s.variant, _flags(s), precision
+        d.Nx, d.Ny, d.variant, d.precision = s.Nx, s.Ny, s.variant, precision
+        d.flags = _flags(s) | (FP32_LOW_PART if low_part else 0)
         for name in ("G", "Ah", "dt", "A", "dLambda", "dTheta", "RHO0", "minD", "lbc", "ab_chi"):
             setattr(d, name, float(getattr(s, name)))
         d.tau_scale = float(s.tau_scale) if s.tau_scale is not None else 0.0
