@@ -186,7 +186,7 @@ def bench_ensemble(args):
                      "traffic": None, "bytes_per_step": bytes_f32},
         "fp64": {"value": v64, "ms_per_step": ms64 / args.steps,
                  "achieved_GBs": bytes_f64 / (ms64 * 1e-3 / args.steps) / 1e9},
-        "autotune": {"f32": tune32, "f64": tune64, "shapes": ["64x4", "32x8"]},
+        "autotune": {"f32": tune32, "f64": tune64, "shapes": ["64x4", "32x8", "64x4 queue", "32x8 queue"]},
         "plain_fp64_c2_value": plain,
         "fp32_vs_plain_fp64": v32 / plain,
         "fp32_error_vs_fp64": {"eta_rel_maxnorm_max": max(err), "eta_rel_maxnorm_median": float(np.median(err)),

@@ -294,9 +294,14 @@ int swm_gpu_batch_detach(swm_batch *b, swm_ctx *c);
 int swm_gpu_batch_set_steps(swm_batch *b, swm_ctx *c, int64_t nsteps);
 int swm_gpu_batch_remaining(swm_batch *b, const swm_ctx *c, int64_t *remaining);
 int swm_gpu_batch_step(swm_batch *b, int64_t nsteps, int32_t *active);
-/* batch tile shape: 0 = 64x4, 1 = 32x8 (default); batch_tune times `steps`
- * steps of each from a device snapshot of every member, restores the members
- * exactly, selects the fastest (times_ms: 2 per-step times, may be NULL). */
+/* batch tile and launch form, 0..swm_gpu_batch_shape_count()-1: 0 = 64x4 and
+ * 1 = 32x8 (default) tiles with one block per tile; 2 = 64x4 and 3 = 32x8
+ * tiles drained by a persistent grid of resident blocks from per-XCD tile
+ * queues (one launch per step either way; every form gives the same results
+ * bit for bit).  batch_tune times `steps` steps of each from a device snapshot
+ * of every member, restores the members exactly, selects the fastest
+ * (times_ms: shape_count per-step times, may be NULL). */
+int32_t swm_gpu_batch_shape_count(void);
 int swm_gpu_batch_set_shape(swm_batch *b, int32_t shape);
 int swm_gpu_batch_tune(swm_batch *b, int32_t steps, int32_t *best_shape, double *times_ms);
 /* device time of the last swm_gpu_batch_step (ms) */

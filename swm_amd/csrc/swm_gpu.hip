@@ -2289,20 +2289,23 @@ struct swm_batch {
     size_t cap = 0;
     int* downer = nullptr;               // member of each work item
     size_t owner_cap = 0;
+    int* dqueue = nullptr;               // the persistent kernel's per-XCD tile-queue counters
     int nwork = 0, ntrc_max = 0;
     hipGraphExec_t graph = nullptr;
     bool use_graph = true;               // SWM_GPU_GRAPH=0: launch each step directly (profiling)
-    int shape = 1;                       // batch tile shape: kBatchShapes[shape] (32x8: members are
-                                         // small grids, where 32x8 tiles win)
+    int shape = 1;                       // kBatchShapes[shape]: tile and launch form (32x8 one block
+                                         // per tile until a tuning run picks another)
     double last_ms = 0.0;
     std::string err;
 };
 
 namespace {
 
-struct BatchShape { int tx, ty; };
-constexpr BatchShape kBatchShapes[] = {{64, 4}, {32, 8}};
-constexpr int kNumBatchShapes = 2;
+// Tile and launch form of a batched step: one block per tile, or the
+// persistent tile queue (a resident grid that drains per-XCD queues).
+struct BatchShape { int tx, ty, queue; };
+constexpr BatchShape kBatchShapes[] = {{64, 4, 0}, {32, 8, 0}, {64, 4, 1}, {32, 8, 1}};
+constexpr int kNumBatchShapes = 4;
 
 int bfail(swm_batch* b, const std::string& msg, int code = 2) {
     if (b) b->err = msg;
@@ -2396,23 +2399,50 @@ int batch_build_t(swm_batch* b) {
     return 0;
 }
 
-template <typename T, int V, int TX, int TY>
-void batch_launch_s(swm_batch* b) {
-    dim3 grid(b->nwork), block(256);
+// Resident blocks of the persistent batch kernel: its occupancy per CU times
+// the CU count, found once per instantiation.
+template <typename T, int V, int NT, int TX, int TY>
+int queue_grid() {
+    static const int g = [] {
+        int dev = 0, ncu = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, swm::batch_queue_kernel<T, V, NT, TX, TY>, 256, 0);
+        return std::max(1, ncu) * std::max(1, per);
+    }();
+    return g;
+}
+
+template <typename T, int V, int NT, int TX, int TY>
+void batch_launch_k(swm_batch* b, bool queue) {
     const swm::Member<T>* m = (const swm::Member<T>*)b->dmem;
+    if (queue) {
+        const int g = std::min(b->nwork, queue_grid<T, V, NT, TX, TY>());
+        hipLaunchKernelGGL((swm::batch_queue_kernel<T, V, NT, TX, TY>), dim3(g), dim3(256), 0, b->stream, m,
+                           b->dstep, b->downer, b->nwork, b->dqueue);
+    } else {
+        hipLaunchKernelGGL((swm::batch_kernel<T, V, NT, TX, TY>), dim3(b->nwork), dim3(256), 0, b->stream, m,
+                           b->dstep, b->downer, b->nwork);
+    }
+}
+
+template <typename T, int V, int TX, int TY>
+void batch_launch_s(swm_batch* b, bool queue) {
     switch (b->ntrc_max) {
-    case 0: hipLaunchKernelGGL((swm::batch_kernel<T, V, 0, TX, TY>), grid, block, 0, b->stream, m, b->dstep, b->downer, b->nwork); break;
-    case 1: hipLaunchKernelGGL((swm::batch_kernel<T, V, 1, TX, TY>), grid, block, 0, b->stream, m, b->dstep, b->downer, b->nwork); break;
-    default: hipLaunchKernelGGL((swm::batch_kernel<T, V, 2, TX, TY>), grid, block, 0, b->stream, m, b->dstep, b->downer, b->nwork); break;
+    case 0: batch_launch_k<T, V, 0, TX, TY>(b, queue); break;
+    case 1: batch_launch_k<T, V, 1, TX, TY>(b, queue); break;
+    default: batch_launch_k<T, V, 2, TX, TY>(b, queue); break;
     }
 }
 
 template <typename T, int V>
 void batch_launch_v(swm_batch* b) {
     const int nmem = (int)b->mem.size();
-    if (b->shape == 1) batch_launch_s<T, V, 32, 8>(b);
-    else batch_launch_s<T, V, 64, 4>(b);
-    hipLaunchKernelGGL(swm::batch_advance, dim3((nmem + 255) / 256), dim3(256), 0, b->stream, b->dstep, nmem);
+    const BatchShape bs = kBatchShapes[b->shape];
+    if (bs.tx == 32) batch_launch_s<T, V, 32, 8>(b, bs.queue != 0);
+    else batch_launch_s<T, V, 64, 4>(b, bs.queue != 0);
+    hipLaunchKernelGGL(swm::batch_advance, dim3((std::max(nmem, swm::kBatchQueues) + 255) / 256), dim3(256), 0,
+                       b->stream, b->dstep, nmem, b->dqueue);
 }
 
 void batch_launch(swm_batch* b) {
@@ -2442,8 +2472,11 @@ int swm_gpu_batch_create(swm_batch** out) {
     swm_batch* b = new swm_batch();
     if (const char* e = getenv("SWM_GPU_GRAPH")) b->use_graph = atoi(e) != 0;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
-        g_create_err = "batch: could not create a HIP stream/events";
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+        hipMalloc((void**)&b->dqueue, swm::kBatchQueues * sizeof(int)) != hipSuccess ||
+        hipMemsetAsync(b->dqueue, 0, swm::kBatchQueues * sizeof(int), b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess) {
+        g_create_err = "batch: could not create a HIP stream/events or the tile-queue counters";
         swm_gpu_batch_destroy(b);
         return 3;
     }
@@ -2662,6 +2695,8 @@ int swm_gpu_batch_tune(swm_batch* b, int32_t steps, int32_t* best_shape, double*
     return 0;
 }
 
+int32_t swm_gpu_batch_shape_count(void) { return kNumBatchShapes; }
+
 int swm_gpu_batch_last_timing(const swm_batch* b, double* total_ms) {
     if (!b || !total_ms) return 1;
     *total_ms = b->last_ms;
@@ -2680,6 +2715,7 @@ void swm_gpu_batch_destroy(swm_batch* b) {
     if (b->dmem) (void)hipFree(b->dmem);
     if (b->dstep) (void)hipFree(b->dstep);
     if (b->downer) (void)hipFree(b->downer);
+    if (b->dqueue) (void)hipFree(b->dqueue);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);

@@ -1298,15 +1298,10 @@ struct MemberStep {
     int cur, ring, cbuf, pad;
 };
 
-template <typename T, int VAR, int NTRC_MAX, int TX, int TY>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
-batch_kernel(const Member<T>* __restrict__ mem, const MemberStep* __restrict__ stp, const int* __restrict__ owner,
-             int nwork) {
-    using DD = LdsDuDv<T, TX, TY>;
-    using RTC = RowTabC<TX, TY, NTRC_MAX>;
-    __shared__ TileLds<T, (NTRC_MAX > 0 ? NTRC_MAX : 1), (TX + 2) * (TY + 2), TY + 2, 1, DD::DUE, DD::DVE, RTC::LR,
-                       RTC::LK> lds;
-    const int w = xcd_tile(blockIdx.x, nwork);
+// Work item w of a batched step: tile (w - wbase) of member owner[w].
+template <typename T, int VAR, int NTRC_MAX, int TX, int TY, typename L>
+__device__ __forceinline__ void batch_item(const Member<T>* __restrict__ mem, const MemberStep* __restrict__ stp,
+                                           const int* __restrict__ owner, const int w, L& lds) {
     const int lo = owner[w];   // member whose work range holds w
     const Member<T>& m = mem[lo];
     const MemberStep ms = stp[lo];
@@ -1340,10 +1335,71 @@ batch_kernel(const Member<T>* __restrict__ mem, const MemberStep* __restrict__ s
     }
 }
 
+// One block per work item; blocks of one XCD take a contiguous run of items
+// (neighbouring tiles of a member share halo lines in that XCD's L2).
+template <typename T, int VAR, int NTRC_MAX, int TX, int TY>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+batch_kernel(const Member<T>* __restrict__ mem, const MemberStep* __restrict__ stp, const int* __restrict__ owner,
+             int nwork) {
+    using DD = LdsDuDv<T, TX, TY>;
+    using RTC = RowTabC<TX, TY, NTRC_MAX>;
+    __shared__ TileLds<T, (NTRC_MAX > 0 ? NTRC_MAX : 1), (TX + 2) * (TY + 2), TY + 2, 1, DD::DUE, DD::DVE, RTC::LR,
+                       RTC::LK> lds;
+    batch_item<T, VAR, NTRC_MAX, TX, TY>(mem, stp, owner, xcd_tile(blockIdx.x, nwork), lds);
+}
+
+// Persistent tile queue: a grid of resident blocks drains the step's work
+// items through eight counters, one per XCD.  Queue x holds the contiguous run
+// of items that xcd_tile gives XCD x; a block (dispatched to XCD blockIdx % 8)
+// takes items from its own queue and, when that is empty, from the next ones
+// in turn, so uneven members (tracers, tile counts) do not leave an XCD idle
+// while another still has a tail.  A claim takes kQueueChunk items, and the
+// block issues its next claim before it works through the current one, so the
+// atomic's round trip overlaps the tiles.  Every block ends after finding all
+// eight queues empty.  batch_advance clears the counters for the next step, so
+// a captured step graph holds no memset.
+constexpr int kBatchQueues = 8;
+constexpr int kQueueChunk = 2;
+template <typename T, int VAR, int NTRC_MAX, int TX, int TY>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 4)))
+batch_queue_kernel(const Member<T>* __restrict__ mem, const MemberStep* __restrict__ stp,
+                   const int* __restrict__ owner, int nwork, int* __restrict__ queue) {
+    using DD = LdsDuDv<T, TX, TY>;
+    using RTC = RowTabC<TX, TY, NTRC_MAX>;
+    __shared__ TileLds<T, (NTRC_MAX > 0 ? NTRC_MAX : 1), (TX + 2) * (TY + 2), TY + 2, 1, DD::DUE, DD::DVE, RTC::LR,
+                       RTC::LK> lds;
+    __shared__ int s_item;
+    const int q0 = nwork >> 3, r0 = nwork & 7;
+    int x = blockIdx.x & 7;
+    int claim = 0;   // thread 0: the claim in flight on queue x
+    if (threadIdx.x == 0) claim = atomicAdd(&queue[x], kQueueChunk);
+    for (int empty = 0; empty < kBatchQueues;) {
+        __syncthreads();   // the previous item is done with LDS and s_item
+        if (threadIdx.x == 0) s_item = claim;
+        __syncthreads();
+        const int k = s_item;
+        const int lo = x * q0 + (x < r0 ? x : r0);
+        const int n = q0 + (x < r0 ? 1 : 0);
+        if (k >= n) {   // uniform: this queue is drained; claim from the next one
+            x = (x + 1) & 7;
+            ++empty;
+            if (threadIdx.x == 0 && empty < kBatchQueues) claim = atomicAdd(&queue[x], kQueueChunk);
+            continue;
+        }
+        if (threadIdx.x == 0) claim = atomicAdd(&queue[x], kQueueChunk);   // next claim, in flight meanwhile
+        const int kend = k + kQueueChunk < n ? k + kQueueChunk : n;
+        for (int kk = k; kk < kend; ++kk) {
+            if (kk > k) __syncthreads();   // the previous tile is done with LDS
+            batch_item<T, VAR, NTRC_MAX, TX, TY>(mem, stp, owner, lo + kk, lds);
+        }
+    }
+}
+
 // model_advance of every member that stepped: N0 <- N0p1, G ring moves on.
 __global__ void __launch_bounds__(256)
-batch_advance(MemberStep* stp, int nmem) {
+batch_advance(MemberStep* stp, int nmem, int* queue) {
     const int k = blockIdx.x * 256 + threadIdx.x;
+    if (queue && k < kBatchQueues) queue[k] = 0;   // the tile queues of the next step start empty
     if (k >= nmem) return;
     MemberStep s = stp[k];
     if (s.remaining <= 0) return;

@@ -38,7 +38,7 @@ EXPORTED = [
     "swm_gpu_batch_destroy", "swm_gpu_kernel_shape_count", "swm_gpu_set_kernel_shape", "swm_gpu_tune",
     "swm_gpu_set_sor_check_every", "swm_gpu_batch_set_shape", "swm_gpu_batch_tune", "swm_gpu_avg_add",
     "swm_gpu_avg_read", "swm_gpu_add_forcing_stream", "swm_gpu_sums_noalloc", "swm_gpu_elliptic_stats",
-    "swm_gpu_add_dyn_stream",
+    "swm_gpu_add_dyn_stream", "swm_gpu_batch_shape_count",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -142,6 +142,8 @@ def load_library(path: str = LIB_PATH):
     L.swm_gpu_batch_destroy.restype = None
     L.swm_gpu_kernel_shape_count.argtypes = []
     L.swm_gpu_kernel_shape_count.restype = C.c_int
+    L.swm_gpu_batch_shape_count.argtypes = []
+    L.swm_gpu_batch_shape_count.restype = C.c_int32
     L.swm_gpu_set_kernel_shape.argtypes = [p, C.c_int32]
     L.swm_gpu_set_kernel_shape.restype = C.c_int
     L.swm_gpu_tune.argtypes = [p, C.c_int32, C.POINTER(C.c_int32), _dp]
@@ -480,9 +482,10 @@ class SwmBatch:
         self._check(self.L.swm_gpu_batch_set_shape(self._h, int(shape)), "swm_gpu_batch_set_shape")
 
     def tune(self, steps: int = 10):
-        """Time each batch tile shape from a snapshot of every member (restored
-        exactly); selects the fastest.  Returns (best_shape, per-step ms)."""
-        times = (C.c_double * 2)()
+        """Time each batch tile and launch form from a snapshot of every member
+        (restored exactly); selects the fastest.  Returns (best_shape, per-step
+        ms per form)."""
+        times = (C.c_double * self.L.swm_gpu_batch_shape_count())()
         best = C.c_int32()
         self._check(self.L.swm_gpu_batch_tune(self._h, int(steps), C.byref(best), C.cast(times, _dp)),
                     "swm_gpu_batch_tune")

@@ -54,7 +54,7 @@ def test_batch_tune_restores_members(gpu_lib):
         b.step(4)
         before = [snapshot(m, len(s.tracers)) for m, s in zip(mixed, setups)]
         res = autotune.tune_batch(b, steps=3)
-        assert res["best_shape"] in (0, 1)
+        assert res["best_shape"] in range(4)
         assert all(same_snap(x, snapshot(m, len(s.tracers))) for x, m, s in zip(before, mixed, setups))
         assert [b.remaining(m) for m in mixed] == [26, 26, 26]
         b.step(26)
