@@ -219,13 +219,25 @@ int swm_gpu_avg_read(swm_ctx *c, int32_t avg_id, double *host_out, double *weigh
  * results bit for bit.  swm_gpu_tune times
  * `steps` steps of every shape from a device snapshot of the state, restores
  * the state exactly, selects the fastest and returns it (times_ms, if not
- * NULL, receives kernel_shape_count per-step times).  set_sor_check_every
- * makes the fixed-omega SOR test convergence every `every` sweeps (1..64;
- * the reference mode always tests every sweep). */
+ * NULL, receives kernel_shape_count per-step times); with steps >= 6 it then
+ * times the best shape's steps replayed from captured six-step graphs against
+ * direct launches and keeps the faster launch form (swm_gpu_launch_mode reads
+ * it and both per-step times; -1 for a form not timed).  set_launch_mode
+ * picks it by hand (1: graphs, 0: direct launches, the default).
+ * set_sor_check_every makes the fixed-omega SOR test convergence every `every`
+ * sweeps (1..64; the reference mode always tests every sweep).
+ * set_sor_halo sets the LDS halo of the blocked fixed-omega sweeps in cells:
+ * 2K cells for K sweeps per launch (2, 4 or 8; K must divide the cadence),
+ * 0 for one launch per colour, -1 (default) for the largest of 4, 2, 1 sweeps
+ * that divides the cadence.  The reference mode uses K <= 1.  Every halo gives
+ * the same chi bit for bit. */
 int swm_gpu_kernel_shape_count(void);
 int swm_gpu_set_kernel_shape(swm_ctx *c, int32_t shape);
 int swm_gpu_tune(swm_ctx *c, int32_t steps, int32_t *best_shape, double *times_ms);
+int swm_gpu_set_launch_mode(swm_ctx *c, int32_t graphs);
+int swm_gpu_launch_mode(const swm_ctx *c, int32_t *graphs, double *ms_direct, double *ms_graph);
 int swm_gpu_set_sor_check_every(swm_ctx *c, int32_t every);
+int swm_gpu_set_sor_halo(swm_ctx *c, int32_t halo);
 
 /* computeVelocityPotential (src/calc_lib.f90:500-525) without host copies:
  * B = computeDivergence of the current device velocity (calc_lib.f90:607-622)

@@ -206,7 +206,8 @@ struct swm_ctx {
     std::vector<FStream> streams;   // time-interpolated forcing streams, in processing order
     FStream dyn[3];                 // DYNFROMFILE datasets of u, v, eta (rec null when absent)
     int tile = 0;
-    // Steady steps replayed from captured six-step graphs (SWM_GPU_GRAPH=1).
+    // Steady steps replayed from captured six-step graphs (SWM_GPU_GRAPH=1,
+    // swm_gpu_set_launch_mode, or the choice of swm_gpu_tune).
     // Off by default: on MI355X direct launches measured faster at every grid
     // size from 128^2 to 1024^2 (e.g. 1024^2 FP64: 46.34 against 46.95 us per
     // step; 128^2: 9.38 against 9.89 us), the graphs' per-launch overhead
@@ -218,6 +219,8 @@ struct swm_ctx {
     bool chi_valid[2] = {};      // sor_buf[w][0] holds a solution to warm-start from
     int sor_every = 1;           // fixed-omega mode: convergence test every sor_every sweeps
     bool sor_blocked = true;     // fixed-omega mode: several sweeps per launch when sor_every allows
+    int sor_halo = -1;           // LDS halo of the blocked sweeps (2 x sweeps per launch); -1: from sor_every
+    double ms_direct = -1.0, ms_graph = -1.0;   // per-step times of the last tuning run's launch forms
     struct Avg {                 // averaging buffer of a diag task
         int field;               // 0 u, 1 v, 2 eta, 3 CH of tracer `tracer`
         int tracer;
@@ -1640,7 +1643,12 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
     // or 2 sweeps per launch in fixed-omega mode when the test cadence is a
     // multiple of it, else (and in the reference mode, which tests every sweep
     // with its Chebyshev omega) one sweep per launch.
-    const int K = !c->sor_blocked ? 0 : (ref_mode ? 1 : (every % 4 == 0 ? 4 : (every % 2 == 0 ? 2 : 1)));
+    // sweeps per launch K: a set halo (2K cells), else the largest of 4, 2, 1
+    // that divides the test cadence; the reference mode tests every sweep (K <= 1)
+    int K = c->sor_halo >= 0 ? c->sor_halo / 2 : (every % 4 == 0 ? 4 : (every % 2 == 0 ? 2 : 1));
+    if (!c->sor_blocked) K = 0;
+    if (ref_mode && K > 1) K = 1;
+    if (K > 1 && every % K) return fail(c, "the SOR halo's sweeps per launch must divide the test cadence", 1);
     if (K) {
         // Launch n does sweeps n*K+1 .. (n+1)*K from one buffer into the other
         // (dchi -> alt for even n); its convergence test runs in block 0 of
@@ -2112,10 +2120,37 @@ int swm_gpu_tune(swm_ctx* c, int32_t steps, int32_t* best_shape, double* times_m
         if (times_ms) times_ms[k] = ms / steps;
         if (ms < best) { best = ms; best_k = k; }
     }
+    // launch form of the best shape: replay of the captured six-step graphs
+    // against direct launches (the untimed pass captures, the timed one replays)
+    bool graphs = graph0;
+    c->ms_direct = best / steps;
+    c->ms_graph = -1.0;
+    if (!rc && steps >= 6) {
+        c->tile = best_k;
+        c->use_graph = true;
+        drop_graphs(c);
+        if (steady_ok(c)) {
+            float ms = 0.f;
+            rc = c->fp32 ? step_many<float>(c, steps) : step_many<double>(c, steps);
+            if (!rc && restore() == hipSuccess && hipEventRecord(c->ev0, c->stream) == hipSuccess) {
+                rc = c->fp32 ? step_many<float>(c, steps) : step_many<double>(c, steps);
+                if (!rc && (hipEventRecord(c->ev1, c->stream) != hipSuccess ||
+                            hipEventSynchronize(c->ev1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess || restore() != hipSuccess))
+                    rc = 3;
+            } else if (!rc) {
+                rc = 3;
+            }
+            if (!rc) {
+                c->ms_graph = ms / steps;
+                graphs = ms < best;
+            }
+        }
+    }
     if (rc) (void)restore();
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(snap);
-    c->use_graph = graph0;
+    c->use_graph = rc ? graph0 : graphs;
     c->tile = rc ? tile0 : best_k;
     drop_graphs(c);
     if (rc) return rc == 3 ? fail(c, "tuning run failed: " + c->err, 3) : rc;
@@ -2171,7 +2206,37 @@ int swm_gpu_avg_read(swm_ctx* c, int32_t avg_id, double* host_out, double* weigh
 int swm_gpu_set_sor_check_every(swm_ctx* c, int32_t every) {
     if (!c) return 1;
     if (every < 1 || every > 64) return fail(c, "check cadence must be 1..64 sweeps", 1);
+    if (c->sor_halo > 2 && every % (c->sor_halo / 2))
+        return fail(c, "the cadence must be a multiple of the SOR halo's sweeps per launch", 1);
     c->sor_every = every;
+    return 0;
+}
+
+int swm_gpu_set_sor_halo(swm_ctx* c, int32_t halo) {
+    if (!c) return 1;
+    if (halo != -1 && halo != 0 && halo != 2 && halo != 4 && halo != 8)
+        return fail(c, "SOR halo must be -1 (from the cadence), 0, 2, 4 or 8 cells", 1);
+    if (halo > 2 && c->sor_every % (halo / 2))
+        return fail(c, "the SOR halo's sweeps per launch (halo / 2) must divide the test cadence", 1);
+    c->sor_halo = halo;
+    return 0;
+}
+
+int swm_gpu_set_launch_mode(swm_ctx* c, int32_t graphs) {
+    if (!c) return 1;
+    HipLock lock(g_hip_mu);
+    if (int rc = member_refresh(c)) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->use_graph = graphs != 0;
+    drop_graphs(c);
+    return 0;
+}
+
+int swm_gpu_launch_mode(const swm_ctx* c, int32_t* graphs, double* ms_direct, double* ms_graph) {
+    if (!c || !graphs) return 1;
+    *graphs = c->use_graph ? 1 : 0;
+    if (ms_direct) *ms_direct = c->ms_direct;
+    if (ms_graph) *ms_graph = c->ms_graph;
     return 0;
 }
 

@@ -38,7 +38,8 @@ EXPORTED = [
     "swm_gpu_batch_destroy", "swm_gpu_kernel_shape_count", "swm_gpu_set_kernel_shape", "swm_gpu_tune",
     "swm_gpu_set_sor_check_every", "swm_gpu_batch_set_shape", "swm_gpu_batch_tune", "swm_gpu_avg_add",
     "swm_gpu_avg_read", "swm_gpu_add_forcing_stream", "swm_gpu_sums_noalloc", "swm_gpu_elliptic_stats",
-    "swm_gpu_add_dyn_stream", "swm_gpu_batch_shape_count",
+    "swm_gpu_add_dyn_stream", "swm_gpu_batch_shape_count", "swm_gpu_set_launch_mode", "swm_gpu_launch_mode",
+    "swm_gpu_set_sor_halo",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -144,6 +145,12 @@ def load_library(path: str = LIB_PATH):
     L.swm_gpu_kernel_shape_count.restype = C.c_int
     L.swm_gpu_batch_shape_count.argtypes = []
     L.swm_gpu_batch_shape_count.restype = C.c_int32
+    L.swm_gpu_set_launch_mode.argtypes = [p, C.c_int32]
+    L.swm_gpu_set_launch_mode.restype = C.c_int
+    L.swm_gpu_launch_mode.argtypes = [p, C.POINTER(C.c_int32), _dp, _dp]
+    L.swm_gpu_launch_mode.restype = C.c_int
+    L.swm_gpu_set_sor_halo.argtypes = [p, C.c_int32]
+    L.swm_gpu_set_sor_halo.restype = C.c_int
     L.swm_gpu_set_kernel_shape.argtypes = [p, C.c_int32]
     L.swm_gpu_set_kernel_shape.restype = C.c_int
     L.swm_gpu_tune.argtypes = [p, C.c_int32, C.POINTER(C.c_int32), _dp]
@@ -406,6 +413,21 @@ class SwmGpu:
 
     def set_sor_check_every(self, every: int):
         self._check(self.L.swm_gpu_set_sor_check_every(self._h, int(every)), "swm_gpu_set_sor_check_every")
+
+    def set_sor_halo(self, halo: int):
+        """LDS halo of the blocked fixed-omega SOR sweeps in cells (2 x sweeps
+        per launch: 2, 4, 8; 0 = one launch per colour; -1 = from the cadence)."""
+        self._check(self.L.swm_gpu_set_sor_halo(self._h, int(halo)), "swm_gpu_set_sor_halo")
+
+    def set_launch_mode(self, graphs: bool):
+        """Replay captured six-step graphs (True) or launch each step (False)."""
+        self._check(self.L.swm_gpu_set_launch_mode(self._h, int(bool(graphs))), "swm_gpu_set_launch_mode")
+
+    def launch_mode(self):
+        """(graphs, ms per step direct, ms per step graphs) after a tuning run."""
+        g, a, b = C.c_int32(), C.c_double(), C.c_double()
+        self._check(self.L.swm_gpu_launch_mode(self._h, C.byref(g), C.byref(a), C.byref(b)), "swm_gpu_launch_mode")
+        return bool(g.value), a.value, b.value
 
     def sums(self, tracer: int = -1):
         """FP64 compensated (volume, energy, tracer inventory) sums, cos-weighted

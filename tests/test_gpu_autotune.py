@@ -77,6 +77,58 @@ def test_sor_check_cadence(gpu_lib):
     m.set_sor_check_every(1)
     m.velocity_potential(1.9, 0.0, it8, reset=True)     # exactly it8 sweeps, tested every sweep
     assert same(m.get("CHI"), chi8)
-    res = autotune.tune_sor(m, sweeps=64, cadences=(1, 4))
-    assert res["best_check_every"] in (1, 4)
+    res = autotune.tune_sor(m, sweeps=32, cadences=(1, 4))
+    assert (res["halo"], res["check_every"]) in autotune.sor_candidates(cadences=(1, 4))
+    assert len(res["rounds"]) >= 2 and res["rounds"][-1]["sweeps"] > res["rounds"][0]["sweeps"]
     m.close()
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_sor_halo_widths_give_the_same_chi(gpu_lib, fp32):
+    """One launch per colour (halo 0) and blocked launches of 1, 2, 4 sweeps
+    (halo 2, 4, 8 cells) give chi bit for bit, for a sweep count that is not a
+    multiple of the sweeps per launch; a halo whose sweeps per launch do not
+    divide the cadence is refused."""
+    s = configs.c2_nonlinear(seed=5, N=130, dt=10.0, tracer=False)
+    m = gpu.SwmGpu(s).init_from_setup()
+    m.step(9)
+    m.set_sor_check_every(8)
+    chis = {}
+    for halo in (0, 2, 4, 8):
+        m.set_sor_halo(halo)
+        it, rr, ok = m.velocity_potential(1.9, 0.0, 45, fp32_sweeps=fp32, reset=True)
+        assert it == 45 and not ok
+        chis[halo] = m.get("CHI")
+    for halo in (2, 4, 8):
+        assert same(chis[halo], chis[0]), halo
+    with pytest.raises(gpu.SwmGpuError):
+        m.set_sor_check_every(2)         # halo 8 = 4 sweeps per launch, tested every 2
+    m.set_sor_halo(-1)
+    m.set_sor_check_every(2)
+    with pytest.raises(gpu.SwmGpuError):
+        m.set_sor_halo(8)
+    m.close()
+
+
+def test_tune_chooses_launch_form_and_cache_replays_it(gpu_lib, tmp_path):
+    """swm_gpu_tune also times graph replay against direct launches for the
+    best shape; a second context of the same workload takes both from the
+    tuning cache without timing anything, and both keep FP64 parity."""
+    s = configs.c2_nonlinear(seed=6, N=150, dt=10.0)
+    cache = autotune.TuneCache(str(tmp_path / "tune.json"))
+    a = gpu.SwmGpu(s).init_from_setup()
+    a.step(4)
+    res = autotune.tune_context(a, steps=12, cache=cache)
+    graphs, ms_direct, ms_graph = a.launch_mode()
+    assert not res["cached"] and ms_direct > 0 and ms_graph > 0 and graphs == (ms_graph < ms_direct)
+    b = gpu.SwmGpu(s).init_from_setup()
+    res2 = autotune.tune_context(b, cache=autotune.TuneCache(str(tmp_path / "tune.json")))
+    assert res2["cached"] and res2["best_shape"] == res["best_shape"] and b.launch_mode()[0] == graphs
+    o = oracle.Oracle(s)
+    a.step(20)
+    b.step(24)
+    o.step(24)
+    for f, of in (("SWM_U", "u"), ("SWM_ETA", "eta")):
+        assert same(a.get(f), o.get(of)) and same(b.get(f), o.get(of)), f
+    for m in (a, b, o):
+        m.close()
