@@ -55,7 +55,7 @@ STABLE_DT = 18.0
 def read_traffic(precision: str):
     """HBM bytes per step-kernel launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
-    path = os.path.join(ROOT, "profiles", f"step_kernel_c2_1024_{precision}_pmc.txt")
+    path = os.path.join(ROOT, "profiles", f"step_kernel_c2_1024_{precision}.txt")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -205,7 +205,8 @@ def bench_serve(args):
     reqs = serve.request_stream(seed=0, rate=args.rate, duration=args.serve_seconds, keep_every=100)
     offered = sum(r.cells * r.nsteps for r in reqs) / args.serve_seconds
     srv = serve.Server(precision=gpu.FP32, hbm_budget=200e9, chunk=args.chunk)
-    st = srv.run(reqs, realtime=True)
+    st = srv.run(reqs, realtime=True, progress=lambda p: print(f"serve progress {json.dumps(p)}", file=sys.stderr,
+                                                                 flush=True))
     errs = []
     for r in reqs:
         if not r.keep:
@@ -337,6 +338,8 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default kernel shape and direct launches")
+    ap.add_argument("--no-stable-run", action="store_true",
+                    help="skip the companion run at the stable dt (profiles of the headline run alone)")
     ap.add_argument("--workload", choices=["c2", "ensemble", "serve", "elliptic"], default="c2",
                     help="c2: the 1024 x 1024 basin (default); ensemble: the batched 64-member ensemble; "
                          "serve: the request stream; elliptic: the 2048 x 1024 per-step SOR")
@@ -389,7 +392,7 @@ def main(argv=None):
     finite = bool(np.isfinite(eta).all())
     g.close()
     stable = None
-    if args.dt > STABLE_DT:
+    if args.dt > STABLE_DT and not args.no_stable_run:
         # The same workload at the largest whole-second dt inside AB3's stability
         # region: the step time must not depend on whether the fields overflowed.
         s2 = configs.c2_nonlinear(seed=0, N=args.n, dt=STABLE_DT, nsteps=args.warmup + args.steps)

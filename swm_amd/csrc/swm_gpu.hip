@@ -2354,7 +2354,8 @@ struct swm_batch {
     size_t cap = 0;
     int* downer = nullptr;               // member of each work item
     size_t owner_cap = 0;
-    int* dqueue = nullptr;               // the persistent kernel's per-XCD tile-queue counters
+    int* dqueue = nullptr;               // per-XCD tile-queue counters of the persistent kernel, then the
+                                         // finished-block counter of either kernel (kBatchQueues + 1)
     int nwork = 0, ntrc_max = 0;
     hipGraphExec_t graph = nullptr;
     bool use_graph = true;               // SWM_GPU_GRAPH=0: launch each step directly (profiling)
@@ -2481,13 +2482,15 @@ int queue_grid() {
 template <typename T, int V, int NT, int TX, int TY>
 void batch_launch_k(swm_batch* b, bool queue) {
     const swm::Member<T>* m = (const swm::Member<T>*)b->dmem;
+    const int nmem = (int)b->mem.size();
     if (queue) {
         const int g = std::min(b->nwork, queue_grid<T, V, NT, TX, TY>());
         hipLaunchKernelGGL((swm::batch_queue_kernel<T, V, NT, TX, TY>), dim3(g), dim3(256), 0, b->stream, m,
-                           b->dstep, b->downer, b->nwork, b->dqueue);
+                           b->dstep, b->downer, b->nwork, nmem, b->dqueue);
     } else {
         hipLaunchKernelGGL((swm::batch_kernel<T, V, NT, TX, TY>), dim3(b->nwork), dim3(256), 0, b->stream, m,
                            b->dstep, b->downer, b->nwork);
+        hipLaunchKernelGGL(swm::batch_advance, dim3((nmem + 255) / 256), dim3(256), 0, b->stream, b->dstep, nmem);
     }
 }
 
@@ -2500,14 +2503,13 @@ void batch_launch_s(swm_batch* b, bool queue) {
     }
 }
 
+// One batched step: the queue kernel's last block advances the members; the
+// one-block-per-tile kernel is followed by batch_advance.
 template <typename T, int V>
 void batch_launch_v(swm_batch* b) {
-    const int nmem = (int)b->mem.size();
     const BatchShape bs = kBatchShapes[b->shape];
     if (bs.tx == 32) batch_launch_s<T, V, 32, 8>(b, bs.queue != 0);
     else batch_launch_s<T, V, 64, 4>(b, bs.queue != 0);
-    hipLaunchKernelGGL(swm::batch_advance, dim3((std::max(nmem, swm::kBatchQueues) + 255) / 256), dim3(256), 0,
-                       b->stream, b->dstep, nmem, b->dqueue);
 }
 
 void batch_launch(swm_batch* b) {
@@ -2538,8 +2540,8 @@ int swm_gpu_batch_create(swm_batch** out) {
     if (const char* e = getenv("SWM_GPU_GRAPH")) b->use_graph = atoi(e) != 0;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
-        hipMalloc((void**)&b->dqueue, swm::kBatchQueues * sizeof(int)) != hipSuccess ||
-        hipMemsetAsync(b->dqueue, 0, swm::kBatchQueues * sizeof(int), b->stream) != hipSuccess ||
+        hipMalloc((void**)&b->dqueue, (swm::kBatchQueues + 1) * sizeof(int)) != hipSuccess ||
+        hipMemsetAsync(b->dqueue, 0, (swm::kBatchQueues + 1) * sizeof(int), b->stream) != hipSuccess ||
         hipStreamSynchronize(b->stream) != hipSuccess) {
         g_create_err = "batch: could not create a HIP stream/events or the tile-queue counters";
         swm_gpu_batch_destroy(b);

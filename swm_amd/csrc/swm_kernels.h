@@ -1277,7 +1277,7 @@ step_kernel_p(const Dyn<T> d) {
 // Batched step: many independent domains of uneven size advance by one step in
 // one launch.  Work item w is tile (w - wbase) of member owner[w]; the
 // per-member step state (level, AB ring slot, itt, steps left) lives in
-// device memory and batch_advance moves it on after each step, so a captured
+// device memory and the last block of each step moves it on, so a captured
 // graph of the two launches replays any number of steps.
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -1335,6 +1335,42 @@ __device__ __forceinline__ void batch_item(const Member<T>* __restrict__ mem, co
     }
 }
 
+// model_advance of every member that stepped: N0 <- N0p1, G ring moves on.
+__device__ __forceinline__ void advance_members(MemberStep* stp, int nmem) {
+    for (int k = threadIdx.x; k < nmem; k += 256) {
+        MemberStep s = stp[k];
+        if (s.remaining <= 0) continue;
+        s.itt += 1;
+        s.remaining -= 1;
+        s.cur = 1 - s.cur;
+        s.ring = s.ring == 2 ? 0 : s.ring + 1;
+        s.cbuf = 0;
+        stp[k] = s;
+    }
+}
+
+// The block that finishes last (device-scope counter ctr[kBatchQueues], fenced)
+// advances every member and clears the step's counters, so one launch is a
+// whole batched step: every other block has read its member's step state
+// before it counted itself done.  Used by the persistent queue kernel, whose
+// grid is the resident blocks; with one block per tile (tens of thousands of
+// blocks) the atomics on one counter serialise and cost more than the
+// separate batch_advance launch.
+constexpr int kBatchQueues = 8;
+__device__ __forceinline__ void batch_finish(MemberStep* stp, int nmem, int* ctr) {
+    __shared__ int s_last;
+    __syncthreads();   // the block's work is done
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_last = atomicAdd(&ctr[kBatchQueues], 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    advance_members(stp, nmem);
+    if (threadIdx.x <= kBatchQueues) ctr[threadIdx.x] = 0;   // tile queues and this counter
+}
+
 // One block per work item; blocks of one XCD take a contiguous run of items
 // (neighbouring tiles of a member share halo lines in that XCD's L2).
 template <typename T, int VAR, int NTRC_MAX, int TX, int TY>
@@ -1348,6 +1384,12 @@ batch_kernel(const Member<T>* __restrict__ mem, const MemberStep* __restrict__ s
     batch_item<T, VAR, NTRC_MAX, TX, TY>(mem, stp, owner, xcd_tile(blockIdx.x, nwork), lds);
 }
 
+// model_advance after a batch_kernel step (one thread per member)
+__global__ void __launch_bounds__(256) batch_advance(MemberStep* stp, int nmem) {
+    const int b0 = blockIdx.x * 256;   // this block's members: [b0, b0 + 256)
+    if (b0 < nmem) advance_members(stp + b0, nmem - b0 < 256 ? nmem - b0 : 256);
+}
+
 // Persistent tile queue: a grid of resident blocks drains the step's work
 // items through eight counters, one per XCD.  Queue x holds the contiguous run
 // of items that xcd_tile gives XCD x; a block (dispatched to XCD blockIdx % 8)
@@ -1356,14 +1398,13 @@ batch_kernel(const Member<T>* __restrict__ mem, const MemberStep* __restrict__ s
 // while another still has a tail.  A claim takes kQueueChunk items, and the
 // block issues its next claim before it works through the current one, so the
 // atomic's round trip overlaps the tiles.  Every block ends after finding all
-// eight queues empty.  batch_advance clears the counters for the next step, so
-// a captured step graph holds no memset.
-constexpr int kBatchQueues = 8;
+// eight queues empty.  The last block to finish clears the counters for the
+// next step (batch_finish), so a captured step graph holds no memset.
 constexpr int kQueueChunk = 2;
 template <typename T, int VAR, int NTRC_MAX, int TX, int TY>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 4)))
-batch_queue_kernel(const Member<T>* __restrict__ mem, const MemberStep* __restrict__ stp,
-                   const int* __restrict__ owner, int nwork, int* __restrict__ queue) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+batch_queue_kernel(const Member<T>* __restrict__ mem, MemberStep* __restrict__ stp,
+                   const int* __restrict__ owner, int nwork, int nmem, int* __restrict__ queue) {
     using DD = LdsDuDv<T, TX, TY>;
     using RTC = RowTabC<TX, TY, NTRC_MAX>;
     __shared__ TileLds<T, (NTRC_MAX > 0 ? NTRC_MAX : 1), (TX + 2) * (TY + 2), TY + 2, 1, DD::DUE, DD::DVE, RTC::LR,
@@ -1393,23 +1434,9 @@ batch_queue_kernel(const Member<T>* __restrict__ mem, const MemberStep* __restri
             batch_item<T, VAR, NTRC_MAX, TX, TY>(mem, stp, owner, lo + kk, lds);
         }
     }
+    batch_finish(stp, nmem, queue);
 }
 
-// model_advance of every member that stepped: N0 <- N0p1, G ring moves on.
-__global__ void __launch_bounds__(256)
-batch_advance(MemberStep* stp, int nmem, int* queue) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (queue && k < kBatchQueues) queue[k] = 0;   // the tile queues of the next step start empty
-    if (k >= nmem) return;
-    MemberStep s = stp[k];
-    if (s.remaining <= 0) return;
-    s.itt += 1;
-    s.remaining -= 1;
-    s.cur = 1 - s.cur;
-    s.ring = s.ring == 2 ? 0 : s.ring + 1;
-    s.cbuf = 0;
-    stp[k] = s;
-}
 
 // ---------------------------------------------------------------------------
 // Marching-wavefront step kernel.

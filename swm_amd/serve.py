@@ -153,9 +153,12 @@ class Server:
             self.used -= fp
             self.cv.notify_all()
 
-    def run(self, requests: List[Request], realtime: bool = True) -> dict:
+    def run(self, requests: List[Request], realtime: bool = True, progress: Optional[Callable[[dict], None]] = None,
+            progress_every: float = 30.0) -> dict:
         """Serve every request; returns summary statistics.  With realtime
-        False all requests are available at once (arrival order kept)."""
+        False all requests are available at once (arrival order kept).
+        progress, if given, receives a status dict every progress_every
+        seconds of wall time (requests done and active, chunks, GPU time)."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         ready: "queue.Queue" = queue.Queue(maxsize=self.max_ready)
         stop = threading.Event()
@@ -166,6 +169,8 @@ class Server:
         active = {}             # id(ctx) -> (request, ctx, footprint)
         produced_all = False
         cell_steps = 0
+        done = 0
+        t_report = progress_every
         try:
             while True:
                 # attach every handed-over context (block only when the GPU is idle)
@@ -209,7 +214,12 @@ class Server:
                         self._release(fp)
                         r.t_done = now
                         cell_steps += r.cells * r.nsteps
+                        done += 1
                         del active[key]
+                if progress and now >= t_report:
+                    t_report = now + progress_every
+                    progress({"t_s": now, "done": done, "active": len(active), "requests": len(reqs),
+                              "chunks": self.chunks, "gpu_busy_s": self.gpu_ms * 1e-3})
         finally:
             stop.set()
             with self.cv:

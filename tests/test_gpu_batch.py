@@ -81,6 +81,31 @@ def test_every_batch_form_equals_alone(gpu_lib, shape):
             m.close()
 
 
+@pytest.mark.parametrize("shape", [1, 3])
+def test_batch_of_more_members_than_one_block(gpu_lib, shape):
+    """300 small members (the step state of more than one 256-thread block of
+    the member advance), one block per tile and the persistent queue: every
+    member advances exactly its own steps, equal to itself alone."""
+    setups = [host.finalize(configs.basin(32 + 16 * (k % 3), 16 + 16 * (k % 2), 5e3, 5e3,
+                                          variant=host.FULLY_NONLINEAR, H0=300.0 + k, dt=5.0, Ah=50.0, f0=1e-4,
+                                          beta=2e-11, lateral_mixing=True, seed=100 + k)) for k in range(300)]
+    mixed = [gpu.SwmGpu(s).init_from_setup() for s in setups]
+    with gpu.SwmBatch() as b:
+        b.set_shape(shape)
+        for k, m in enumerate(mixed):
+            b.attach(m, 5 + k % 7)
+        b.step(3)
+        b.step(20)
+        assert all(m.itt == 5 + k % 7 for k, m in enumerate(mixed))
+    for k in (0, 1, 255, 256, 257, 299):
+        a = gpu.SwmGpu(setups[k]).init_from_setup()
+        a.step(5 + k % 7)
+        check_same(mixed[k], a, 0, f"member {k}")
+        a.close()
+    for m in mixed:
+        m.close()
+
+
 def test_batch_matches_oracle_and_resumes_alone(gpu_lib):
     """A member's batched run is bit-identical to the oracle; after detach it
     continues alone, and re-attaching with a new count continues the batch."""

@@ -22,10 +22,10 @@ HOT = [
     "_ZN3swm12batch_kernelIfLi1ELi1ELi32ELi8EEEvPKNS_6MemberIT_EEPKNS_10MemberStepEPKii",
     "_ZN3swm12batch_kernelIdLi1ELi0ELi32ELi8EEEvPKNS_6MemberIT_EEPKNS_10MemberStepEPKii",
     "_ZN3swm12batch_kernelIfLi1ELi0ELi32ELi8EEEvPKNS_6MemberIT_EEPKNS_10MemberStepEPKii",
-    "_ZN3swm18batch_queue_kernelIdLi1ELi1ELi32ELi8EEEvPKNS_6MemberIT_EEPKNS_10MemberStepEPKiiPi",
-    "_ZN3swm18batch_queue_kernelIfLi1ELi1ELi32ELi8EEEvPKNS_6MemberIT_EEPKNS_10MemberStepEPKiiPi",
-    "_ZN3swm18batch_queue_kernelIdLi1ELi1ELi64ELi4EEEvPKNS_6MemberIT_EEPKNS_10MemberStepEPKiiPi",
-    "_ZN3swm18batch_queue_kernelIfLi1ELi1ELi64ELi4EEEvPKNS_6MemberIT_EEPKNS_10MemberStepEPKiiPi",
+    "_ZN3swm18batch_queue_kernelIdLi1ELi1ELi32ELi8EEEvPKNS_6MemberIT_EEPNS_10MemberStepEPKiiiPi",
+    "_ZN3swm18batch_queue_kernelIfLi1ELi1ELi32ELi8EEEvPKNS_6MemberIT_EEPNS_10MemberStepEPKiiiPi",
+    "_ZN3swm18batch_queue_kernelIdLi1ELi1ELi64ELi4EEEvPKNS_6MemberIT_EEPNS_10MemberStepEPKiiiPi",
+    "_ZN3swm18batch_queue_kernelIfLi1ELi1ELi64ELi4EEEvPKNS_6MemberIT_EEPNS_10MemberStepEPKiiiPi",
 ]
 
 
