@@ -12,6 +12,9 @@ of them:
 * on the sphere, a solid-body rotation u = U cos(theta) has no strain, so its
   mixing tendency vanishes as the grid is refined, at second order: the tan
   terms of the coefficients cancel the curvature of cos(theta);
+* on a spherical band symmetric about the equator with mirrored coasts, a
+  flow with u even and v odd in latitude has a mixing trend of the same
+  parities: the metric terms and their coastal mask weights mirror;
 * tracer (tracer_module.f90:154-209, 291-342): diffusion of a mode with no
   flow and advection of a mode by a uniform flow have exact discrete symbols
   (-kappa h 4 sin^2(k/2)/dx^2 and -U h sin(k)/dx times the shifted mode); a
@@ -287,6 +290,44 @@ def test_mixing_operator_symmetric_and_dissipative(lbc):
     ab, ba, aa = dot(a, Lb), dot(b, La), dot(a, La)
     assert abs(ab - ba) <= 1e-12 * abs(aa)
     assert aa < 0.0 and dot(b, Lb) < 0.0
+
+
+@pytest.mark.parametrize("lbc", [0.0, 2.0])
+def test_mixing_mirror_symmetric_about_the_equator(lbc):
+    """On a spherical band symmetric about the equator, with land (coasts,
+    islands) mirrored onto itself, a flow with u even and v odd in latitude
+    has a mixing trend with the same parities: the metric (tan) terms and the
+    ocean masks that weight them at a southern coast must mirror those at the
+    northern one.  H and v rows j mirror onto Ny-1-j, eta and u rows onto
+    Ny-2-j (the last eta row is land)."""
+    n, ny = 40, 36
+    rng = np.random.default_rng(3)
+    Hin = np.full((ny, n), 500.0)
+    Hin[:, 0] = 0.0
+    Hin[:, -1] = 0.0
+    for _ in range(8):                                  # islands of 3x3 land corners
+        i, j = rng.integers(2, n - 5), rng.integers(2, ny - 5)
+        Hin[j:j + 3, i:i + 3] = 0.0
+    Hin = np.where(np.arange(ny)[:, None] < ny // 2, Hin, Hin[::-1])
+    grids, dL, dT = host.init_domain(n, ny, 0.0, 30.0, -30.0, 30.0, H_in=Hin, lbc=lbc, coriolis_approx=0)
+    s = host.Setup(Nx=n, Ny=ny, variant=host.FULLY_NONLINEAR, grids=grids, dLambda=dL, dTheta=dT,
+                   lateral_mixing=True, barotropic=True, Ah=1e4, dt=10.0, lbc=lbc)
+    host.init_damping(s, [])
+    e, v_ = slice(0, ny - 1), slice(1, ny - 1)
+    assert np.abs(grids["eta"].lat[e] + grids["eta"].lat[e][::-1]).max() < 1e-9
+    assert np.array_equal(grids["v"].ocean[v_], grids["v"].ocean[v_][::-1])
+    # the band's first and last v rows are land: coasts along both edges, and islands inside
+    assert not grids["v"].ocean[0].any() and not grids["v"].ocean[ny - 1].any()
+    assert (grids["eta"].ocean[1:ny - 2, 1:n - 1] == 0).any()
+    u = rng.standard_normal((ny, n))
+    u[e] = 0.5 * (u[e] + u[e][::-1])
+    v = rng.standard_normal((ny, n))
+    v[v_] = 0.5 * (v[v_] - v[v_][::-1])
+    v[0] = v[ny - 1] = 0.0
+    lu, lv = latmix_of(s, u0=u, v0=v)
+    scale = max(np.abs(lu).max(), np.abs(lv).max())
+    assert np.abs(lu[e] - lu[e][::-1]).max() <= 1e-13 * scale
+    assert np.abs(lv[v_] + lv[v_][::-1]).max() <= 1e-13 * scale
 
 
 @pytest.mark.parametrize("lbc", [0.0, 2.0])
