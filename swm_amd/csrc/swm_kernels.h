@@ -783,6 +783,90 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
     const T* const Ge2 = st + (PL_GE + r1) * pl;
 
     // ---- phase 2: derived fields on the extended tile
+    constexpr bool SPLIT2 = !DIAG && !LdsDuDv<T, TX, TY>::SDU && !LdsDuDv<T, TX, TY>::SDV;
+    if constexpr (SPLIT2) {
+        // Two loops over the two regions the stencils need, instead of one loop
+        // over the whole (TX+2)(TY+2) tile: EDens and P_lambda-lambda on
+        // lx 0..TX, ly 0..TY, then (Pot overwrites eta in place) Dh, zeta, Pot
+        // and P_lambda-theta on lx 1..TX+1, ly 1..TY+1.  Each has
+        // (TX+1)(TY+1) cells; their second passes run on different waves
+        // (tid ^ 128: wave 2 for the first loop, wave 0 for the second; wave 3
+        // for the second measured 0.5 % slower).
+        constexpr int RW = TX + 1, NR = (TX + 1) * (TY + 1);
+        for (int r = tid ^ 128; r < NR; r += NT) {
+            const int lx = r % RW, ly = r / RW;
+            const int e = ly * EW + lx;
+            const uint8_t m = s_m[e];
+            const int ke = e + 1, kn = e + EW;
+            T Ed = T(0);
+            if (OE(m)) {
+                if (FN) {
+                    const T ue = s_u[ke], uc = s_u[e], vn = s_v[kn], vc = s_v[e];
+                    Ed = ((T(0.5) * (ue * ue) + T(0.5) * (uc * uc)) + (T(0.5) * (vn * vn) + T(0.5) * (vc * vc))) / T(2) +
+                         a.G * s_eta[e];
+                } else if (VAR == VAR_MS) {   // g*eta + uU + vV
+                    const int gj = wrap(j0 + ly - 1, Ny);
+                    const int gi = wrap(i0 + lx - 1, Nx), gip = gi + 1 == Nx ? 0 : gi + 1;
+                    const int gjp = gj + 1 == Ny ? 0 : gj + 1;
+                    const T ube = fldg(a.ubs, (size_t)gj * P + gip, gj, OU(s_m[ke]) != 0);
+                    const T ubc = fldg(a.ubs, (size_t)gj * P + gi, gj, OU(m) != 0);
+                    const T vbn = fldg(a.vbs, (size_t)gjp * P + gi, gjp, OV(s_m[kn]) != 0);
+                    const T vbc = fldg(a.vbs, (size_t)gj * P + gi, gj, OV(m) != 0);
+                    const T ue = s_u[ke], uc = s_u[e], vn = s_v[kn], vc = s_v[e];
+                    Ed = ((T(0.5) * (ue * ube) + T(0.5) * (uc * ubc)) + (T(0.5) * (vn * vbn) + T(0.5) * (vc * vbc))) +
+                         a.G * s_eta[e];
+                } else {
+                    Ed = a.G * s_eta[e];
+                }
+            }
+            s_Ed[e] = Ed;
+            T pll = T(0);
+            if (a.mix && OE(m)) {
+                const T* c = &lds.pllc[ly][(OV(s_m[kn]) * 2 + OV(m)) * 3];
+                pll = s_D[e] * ((c[0] * (s_u[ke] - s_u[e]) + c[1] * s_v[kn]) + c[2] * s_v[e]);
+            }
+            s_pll[e] = pll;
+        }
+        __syncthreads();   // every eta read before Pot takes its place
+        for (int r = tid; r < NR; r += NT) {
+            const int lx = r % RW + 1, ly = r / RW + 1;
+            const int e = ly * EW + lx;
+            const int gj = wrap(j0 + ly - 1, Ny);
+            const uint8_t m = s_m[e];
+            const int kw = e - 1, ks = e - EW, ksw = e - EW - 1;
+            const int m1 = OE(m), m2 = OE(s_m[ksw]), m3 = OE(s_m[kw]), m4 = OE(s_m[ks]);
+            const T w = noland_w<T>(m1 + m2 + m3 + m4);
+            const T Dh = (((m1 ? w : T(0)) * s_D[e] + (m2 ? w : T(0)) * s_D[ksw]) + (m3 ? w : T(0)) * s_D[kw]) +
+                         (m4 ? w : T(0)) * s_D[ks];
+            T zeta = T(0);
+            if (FN || VAR == VAR_MS) {
+                T za_ = T(0), zb = T(0);
+                if (OV(s_m[kw]) + OV(m) != 0)
+                    za_ = lds.za[ly][OH(m) ? 0 : 1] * (s_v[e] - s_v[kw]);
+                if (OU(m) + OU(s_m[ks]) != 0) {
+                    const T bcH = OH(m) ? T(1) : a.lbc;
+                    zb = dvr(bcH * (lds.cu[ly + 1] * s_u[e] - lds.cu[ly] * s_u[ks]), a.AdT, a.rAdT);
+                }
+                zeta = za_ - dvr(zb, lds.cH[ly], lds.rcH[ly]);
+            }
+            const T pD = Dh > a.eps ? Dh : a.eps;
+            T Pot;
+            if constexpr (VAR == VAR_MS) {   // (f + zeta_bs) / pD
+                const int gi = wrap(i0 + lx - 1, Nx);
+                Pot = dv(lds.fH[ly] + fldg(a.zbs, (size_t)gj * P + gi, gj, OH(m)), pD);
+                lds.zt[e] = zeta;
+            } else {
+                Pot = FN ? dv(lds.fH[ly] + zeta, pD) : dv(lds.fH[ly], pD);
+            }
+            s_Pot[e] = Pot;
+            T plt = T(0);
+            if (a.mix && OH(m)) {
+                const T* c = &lds.pltc[ly][(OU(m) * 2 + OU(s_m[ks])) * 3];
+                plt = Dh * ((c[0] * (s_v[e] - s_v[kw]) + c[1] * s_u[e]) + c[2] * s_u[ks]);
+            }
+            s_plt[e] = plt;
+        }
+    } else
     for (int e = tid; e < E; e += NT) {
         const int lx = e % EW, ly = e / EW;
         const int gj = wrap(j0 + ly - 1, Ny);
