@@ -238,6 +238,10 @@ int swm_gpu_set_launch_mode(swm_ctx *c, int32_t graphs);
 int swm_gpu_launch_mode(const swm_ctx *c, int32_t *graphs, double *ms_direct, double *ms_graph);
 int swm_gpu_set_sor_check_every(swm_ctx *c, int32_t every);
 int swm_gpu_set_sor_halo(swm_ctx *c, int32_t halo);
+/* rows of the blocked sweeps' 64-column LDS tile: 16 (default) or 32 (fewer
+ * halo cells per output; not with the 8-cell halo, whose 64x32 tile would
+ * exceed 64 KiB of LDS).  Every tile gives the same chi bit for bit. */
+int swm_gpu_set_sor_tile_rows(swm_ctx *c, int32_t rows);
 
 /* computeVelocityPotential (src/calc_lib.f90:500-525) without host copies:
  * B = computeDivergence of the current device velocity (calc_lib.f90:607-622)

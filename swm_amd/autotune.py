@@ -12,8 +12,10 @@ Search spaces (every candidate gives the same results bit for bit):
   persistent grid draining per-XCD tile queues) - swm_gpu_batch_tune, likewise;
 * the fixed-omega SOR: halo width of the blocked sweeps (0 = one launch per
   colour; 2, 4, 8 cells = 1, 2, 4 sweeps per launch, each sweep consuming two
-  halo cells) x convergence-test cadence (1..16 sweeps), where the sweeps per
-  launch must divide the cadence.  The pairs are raced by successive halving:
+  halo cells) x convergence-test cadence (1..16 sweeps) x rows of the 64-wide
+  LDS tile (16 or 32: fewer halo cells per output, more LDS), where the sweeps
+  per launch must divide the cadence and the 8-cell halo takes 16-row tiles.
+  The candidates are raced by successive halving:
   every survivor solves the same right-hand side from the same zero start for
   a fixed number of sweeps (tolerance 0, so none stops early); the slower half
   is dropped and the sweep count doubled until one pair is left.  The cadence
@@ -113,45 +115,50 @@ def tune_batch(b: "gpu.SwmBatch", steps: int = 10, cache: Optional[TuneCache] = 
     return dict(res, cached=False)
 
 
-def sor_candidates(halos: Iterable[int] = (0, 2, 4, 8), cadences: Iterable[int] = (1, 2, 4, 8, 16)):
-    """(halo, cadence) pairs whose sweeps per launch (halo / 2) divide the cadence."""
-    return [(h, c) for h in halos for c in cadences if h <= 2 or c % (h // 2) == 0]
+def sor_candidates(halos: Iterable[int] = (0, 2, 4, 8), cadences: Iterable[int] = (1, 2, 4, 8, 16),
+                   rows: Iterable[int] = (16, 32)):
+    """(halo, cadence, rows) whose sweeps per launch (halo / 2) divide the
+    cadence; 32-row tiles only for blocked sweeps of at most 2 per launch (the
+    unblocked colour launches have no tile)."""
+    return [(h, c, r) for h in halos for c in cadences for r in rows
+            if (h <= 2 or c % (h // 2) == 0) and (r == 16 or 0 < h <= 4)]
 
 
-def apply_sor(m: "gpu.SwmGpu", halo: int, cadence: int):
-    m.set_sor_halo(-1)            # neither setting may contradict the other in between
+def apply_sor(m: "gpu.SwmGpu", halo: int, cadence: int, rows: int = 16):
+    m.set_sor_halo(-1)            # no setting may contradict another in between
+    m.set_sor_tile_rows(rows)
     m.set_sor_check_every(cadence)
     m.set_sor_halo(halo)
 
 
 def tune_sor(m: "gpu.SwmGpu", omega: float = 1.9, fp32_sweeps: bool = False,
              halos: Sequence[int] = (0, 2, 4, 8), cadences: Sequence[int] = (1, 2, 4, 8, 16),
-             sweeps: int = 64, cache: Optional[TuneCache] = None) -> dict:
-    """Successive halving over (halo, cadence): time `sweeps` fixed-omega
-    sweeps of every candidate, keep the faster half, double the sweeps, until
-    one is left; applies and returns it."""
+             rows: Sequence[int] = (16, 32), sweeps: int = 64, cache: Optional[TuneCache] = None) -> dict:
+    """Successive halving over (halo, cadence, rows): time `sweeps`
+    fixed-omega sweeps of every candidate, keep the faster half, double the
+    sweeps, until one is left; applies and returns it."""
     key = f"{signature(m)}/w{omega}/f{int(bool(fp32_sweeps))}"
     hit = cache.get("sor", key) if cache else None
     if hit is not None:
-        apply_sor(m, hit["halo"], hit["check_every"])
+        apply_sor(m, hit["halo"], hit["check_every"], hit.get("rows", 16))
         return dict(hit, cached=True)
-    cand = sor_candidates(halos, cadences)
+    cand = sor_candidates(halos, cadences, rows)
     rounds = []
     n = sweeps
     while len(cand) > 1:
         times = {}
-        for h, c in cand:
-            apply_sor(m, h, c)
+        for h, c, r in cand:
+            apply_sor(m, h, c, r)
             m.velocity_potential(omega, 0.0, n, fp32_sweeps=fp32_sweeps, reset=True)   # warm caches
             t0 = time.perf_counter()
             m.velocity_potential(omega, 0.0, n, fp32_sweeps=fp32_sweeps, reset=True)
-            times[(h, c)] = (time.perf_counter() - t0) / n * 1e3
-        rounds.append({"sweeps": n, "ms_per_sweep": {f"halo{h}/every{c}": t for (h, c), t in times.items()}})
+            times[(h, c, r)] = (time.perf_counter() - t0) / n * 1e3
+        rounds.append({"sweeps": n, "ms_per_sweep": {f"halo{h}/every{c}/rows{r}": t for (h, c, r), t in times.items()}})
         cand = sorted(cand, key=times.get)[:max(1, len(cand) // 2)]
         n *= 2
-    (h, c), = cand
-    apply_sor(m, h, c)
-    res = {"halo": h, "check_every": c, "sweeps_per_launch": h // 2, "rounds": rounds}
+    (h, c, r), = cand
+    apply_sor(m, h, c, r)
+    res = {"halo": h, "check_every": c, "rows": r, "sweeps_per_launch": h // 2, "rounds": rounds}
     if cache:
         cache.put("sor", key, res)
     return dict(res, cached=False)

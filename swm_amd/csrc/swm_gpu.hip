@@ -220,6 +220,7 @@ struct swm_ctx {
     int sor_every = 1;           // fixed-omega mode: convergence test every sor_every sweeps
     bool sor_blocked = true;     // fixed-omega mode: several sweeps per launch when sor_every allows
     int sor_halo = -1;           // LDS halo of the blocked sweeps (2 x sweeps per launch); -1: from sor_every
+    int sor_rows = 16;           // rows of the blocked sweeps' 64-column tile (16, or 32 with <= 2 sweeps per launch)
     double ms_direct = -1.0, ms_graph = -1.0;   // per-step times of the last tuning run's launch forms
     struct Avg {                 // averaging buffer of a diag task
         int field;               // 0 u, 1 v, 2 eta, 3 CH of tracer `tracer`
@@ -1599,10 +1600,10 @@ int sor_prepare(swm_ctx* c, bool f32, int nblk) {
 
 // Fixed-omega sweeps l .. l+K-1 in one blocked launch (sor_block) from chi_in
 // into chi_out, then the convergence test if sweep l+K-1 is a test sweep.
-template <typename T, int K>
+template <typename T, int K, int TY>
 void sor_block_launch(swm_ctx* c, const void* chi_in, void* chi_out, void* dB, void* const* tabs, double omega,
                       double tol, double bnorm, int64_t l, bool check, bool ref_mode, int parity, int64_t prev_sweep) {
-    constexpr int TX = 64, TY = 16;
+    constexpr int TX = 64;
     const int ntx = (c->Nx + TX - 1) / TX;
     const int nb = ntx * ((c->Ny + TY - 1) / TY);
     swm::SorArgs<T> a{};
@@ -1649,6 +1650,7 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
     if (!c->sor_blocked) K = 0;
     if (ref_mode && K > 1) K = 1;
     if (K > 1 && every % K) return fail(c, "the SOR halo's sweeps per launch must divide the test cadence", 1);
+    const int rows = (K == 4) ? 16 : c->sor_rows;   // 64x32 tiles with a 4-sweep halo exceed 64 KiB of LDS
     if (K) {
         // Launch n does sweeps n*K+1 .. (n+1)*K from one buffer into the other
         // (dchi -> alt for even n); its convergence test runs in block 0 of
@@ -1658,7 +1660,7 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
         const int w = std::is_same<T, float>::value ? 1 : 0;
         void* cur = dchi;
         void* alt = c->sor_alt[w];
-        const int ntx = (c->Nx + 63) / 64, nb = ntx * ((c->Ny + 15) / 16);
+        const int ntx = (c->Nx + 63) / 64, nb = ntx * ((c->Ny + rows - 1) / rows);
         const int64_t nfull = max_sweeps / K * K;
         int64_t n = 0;
         while (l + K - 1 <= nfull && !*host_done) {
@@ -1668,9 +1670,11 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
             for (; l + K - 1 <= lend; l += K, ++n) {
                 const bool check = ((l + K - 1) % every) == 0 || l + K - 1 == max_sweeps;
                 const int par = (int)(n & 1);
-                if (K == 4) sor_block_launch<T, 4>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending);
-                else if (K == 2) sor_block_launch<T, 2>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending);
-                else sor_block_launch<T, 1>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending);
+                if (K == 4) sor_block_launch<T, 4, 16>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending);
+                else if (K == 2 && rows == 32) sor_block_launch<T, 2, 32>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending);
+                else if (K == 2) sor_block_launch<T, 2, 16>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending);
+                else if (rows == 32) sor_block_launch<T, 1, 32>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending);
+                else sor_block_launch<T, 1, 16>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending);
                 pending = check ? l + K - 1 : 0;
                 std::swap(cur, alt);
             }
@@ -2212,10 +2216,19 @@ int swm_gpu_set_sor_check_every(swm_ctx* c, int32_t every) {
     return 0;
 }
 
+int swm_gpu_set_sor_tile_rows(swm_ctx* c, int32_t rows) {
+    if (!c) return 1;
+    if (rows != 16 && rows != 32) return fail(c, "SOR tile rows must be 16 or 32", 1);
+    if (rows == 32 && c->sor_halo == 8) return fail(c, "a 4-sweep halo (8 cells) takes 16-row tiles", 1);
+    c->sor_rows = rows;
+    return 0;
+}
+
 int swm_gpu_set_sor_halo(swm_ctx* c, int32_t halo) {
     if (!c) return 1;
     if (halo != -1 && halo != 0 && halo != 2 && halo != 4 && halo != 8)
         return fail(c, "SOR halo must be -1 (from the cadence), 0, 2, 4 or 8 cells", 1);
+    if (halo == 8 && c->sor_rows == 32) return fail(c, "a 4-sweep halo (8 cells) takes 16-row tiles", 1);
     if (halo > 2 && c->sor_every % (halo / 2))
         return fail(c, "the SOR halo's sweeps per launch (halo / 2) must divide the test cadence", 1);
     c->sor_halo = halo;

@@ -39,7 +39,7 @@ EXPORTED = [
     "swm_gpu_set_sor_check_every", "swm_gpu_batch_set_shape", "swm_gpu_batch_tune", "swm_gpu_avg_add",
     "swm_gpu_avg_read", "swm_gpu_add_forcing_stream", "swm_gpu_sums_noalloc", "swm_gpu_elliptic_stats",
     "swm_gpu_add_dyn_stream", "swm_gpu_batch_shape_count", "swm_gpu_set_launch_mode", "swm_gpu_launch_mode",
-    "swm_gpu_set_sor_halo",
+    "swm_gpu_set_sor_halo", "swm_gpu_set_sor_tile_rows",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -151,6 +151,8 @@ def load_library(path: str = LIB_PATH):
     L.swm_gpu_launch_mode.restype = C.c_int
     L.swm_gpu_set_sor_halo.argtypes = [p, C.c_int32]
     L.swm_gpu_set_sor_halo.restype = C.c_int
+    L.swm_gpu_set_sor_tile_rows.argtypes = [p, C.c_int32]
+    L.swm_gpu_set_sor_tile_rows.restype = C.c_int
     L.swm_gpu_set_kernel_shape.argtypes = [p, C.c_int32]
     L.swm_gpu_set_kernel_shape.restype = C.c_int
     L.swm_gpu_tune.argtypes = [p, C.c_int32, C.POINTER(C.c_int32), _dp]
@@ -418,6 +420,10 @@ class SwmGpu:
         """LDS halo of the blocked fixed-omega SOR sweeps in cells (2 x sweeps
         per launch: 2, 4, 8; 0 = one launch per colour; -1 = from the cadence)."""
         self._check(self.L.swm_gpu_set_sor_halo(self._h, int(halo)), "swm_gpu_set_sor_halo")
+
+    def set_sor_tile_rows(self, rows: int):
+        """Rows of the blocked SOR sweeps' 64-column tile (16 or 32)."""
+        self._check(self.L.swm_gpu_set_sor_tile_rows(self._h, int(rows)), "swm_gpu_set_sor_tile_rows")
 
     def set_launch_mode(self, graphs: bool):
         """Replay captured six-step graphs (True) or launch each step (False)."""

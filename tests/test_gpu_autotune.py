@@ -78,7 +78,7 @@ def test_sor_check_cadence(gpu_lib):
     m.velocity_potential(1.9, 0.0, it8, reset=True)     # exactly it8 sweeps, tested every sweep
     assert same(m.get("CHI"), chi8)
     res = autotune.tune_sor(m, sweeps=32, cadences=(1, 4))
-    assert (res["halo"], res["check_every"]) in autotune.sor_candidates(cadences=(1, 4))
+    assert (res["halo"], res["check_every"], res["rows"]) in autotune.sor_candidates(cadences=(1, 4))
     assert len(res["rounds"]) >= 2 and res["rounds"][-1]["sweeps"] > res["rounds"][0]["sweeps"]
     m.close()
 
@@ -94,13 +94,19 @@ def test_sor_halo_widths_give_the_same_chi(gpu_lib, fp32):
     m.step(9)
     m.set_sor_check_every(8)
     chis = {}
-    for halo in (0, 2, 4, 8):
+    for halo, rows in ((0, 16), (2, 16), (4, 16), (8, 16), (2, 32), (4, 32)):
+        m.set_sor_halo(-1)
+        m.set_sor_tile_rows(rows)
         m.set_sor_halo(halo)
         it, rr, ok = m.velocity_potential(1.9, 0.0, 45, fp32_sweeps=fp32, reset=True)
         assert it == 45 and not ok
-        chis[halo] = m.get("CHI")
-    for halo in (2, 4, 8):
-        assert same(chis[halo], chis[0]), halo
+        chis[(halo, rows)] = m.get("CHI")
+    for key in chis:
+        assert same(chis[key], chis[(0, 16)]), key
+    with pytest.raises(gpu.SwmGpuError):
+        m.set_sor_halo(8)                # 8-cell halo with 32-row tiles
+    m.set_sor_tile_rows(16)
+    m.set_sor_halo(8)
     with pytest.raises(gpu.SwmGpuError):
         m.set_sor_check_every(2)         # halo 8 = 4 sweeps per launch, tested every 2
     m.set_sor_halo(-1)
