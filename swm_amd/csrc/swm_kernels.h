@@ -2098,14 +2098,27 @@ __global__ void __launch_bounds__(256) sor_sweep(const SorArgs<T> a) {
 // same arithmetic).  With accumulate the last sweep's residuals of the output
 // cells (r, or the reference's res) are summed in FP64 in a fixed order into
 // partial[block]; block 0 first runs the previous launch's convergence test.
+//
+// A cell update reads LDS only: the per-row coefficients of the staged rows
+// are staged with the block as a table in which each of c1, c3, c4 sits beside
+// a +0 and is picked by its mask bit as an index (the value sor_sweep's select
+// gives, without a branch around the read).  Where the colours alternate
+// regularly a half sweep walks exactly the cells of its colour inside its
+// shrinking region, and each thread forms the new values of all its cells
+// before it stores any of them, so their LDS reads overlap.
 template <typename T, int TX, int TY, int K>
 __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __restrict__ chi_in,
                                                  T* __restrict__ chi_out, int ntx) {
     constexpr int H = 2 * K, W = TX + 2 * H, HH = TY + 2 * H, E = W * HH;
+    // per staged row: {0, c1}, {0, c3}, {0, c4}, c5[12], 1/c5[12]
+    constexpr int TC1 = 0, TC3 = 2, TC4 = 4, TC5 = 6, TC5R = 18, NC = 30, NL = 27;
+    static_assert(NL == 3 + 2 * 12 && NC == TC5R + 12, "row table layout");
     __shared__ T s_chi[E];
     __shared__ T s_B[E];
-    __shared__ uint8_t s_m[E];   // ocean masks; bit 7 = colour ((i + j) % 2) of the cell
-    __shared__ int s_row[HH];    // global row of each staged row
+    __shared__ T s_c[HH][NC];
+    // per cell: ov | ov(i,j+1) << 1 | (ou(i+1,j) + ou) << 2 | ou(i+1,j) << 4 | ou << 5 |
+    // ocean eta << 6 | colour ((i + j) % 2) << 7: the low four bits index c5
+    __shared__ uint8_t s_m[E];
     __shared__ double red[256 / 64];
     const int Nx = a.Nx, Ny = a.Ny, P = a.P;
     const int i0 = (blockIdx.x % ntx) * TX, j0 = (blockIdx.x / ntx) * TY;
@@ -2119,26 +2132,94 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
                  a.iters, a.norm_out, red, &conv))
         return;
     if (*a.done) return;
-    for (int e = tid; e < E; e += 256) {
-        const int lx = e % W, ly = e / W;
-        const int gi = wrap(i0 - H + lx, Nx), gj = wrap(j0 - H + ly, Ny);
-        const size_t off = (size_t)gj * P + gi;
-        s_chi[e] = chi_in[off];
-        s_B[e] = a.B[off];
-        s_m[e] = (uint8_t)(a.mask[off] | (((gi + gj) & 1) << 7));
-        if (lx == 0) s_row[ly] = gj;
+    // every global load of the stage is issued before the first LDS store, so
+    // the block waits for one memory latency, not one per cell it stages
+    constexpr int NS = (E + 255) / 256, NQ = (HH * NL + 255) / 256;
+    T rchi[NS], rB[NS], rc[NQ];
+    int rm[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        const int e = tid + q * 256;
+        if (e < E) {
+            const int lx = e % W, ly = e / W;
+            const int gi = wrap(i0 - H + lx, Nx), gj = wrap(j0 - H + ly, Ny);
+            const size_t off = (size_t)gj * P + gi;
+            const int gip = gi + 1 == Nx ? 0 : gi + 1, gjp = gj + 1 == Ny ? 0 : gj + 1;
+            rchi[q] = chi_in[off];
+            rB[q] = a.B[off];
+            const int mc = a.mask[off], mip = a.mask[(size_t)gj * P + gip], mjp = a.mask[(size_t)gjp * P + gi];
+            const int ouip = (mip >> 1) & 1, ou = (mc >> 1) & 1, ovjp = (mjp >> 2) & 1, ov = (mc >> 2) & 1;
+            rm[q] = ov | ovjp << 1 | (ouip + ou) << 2 | ouip << 4 | ou << 5 | ((mc & OC_E) ? 1 << 6 : 0) |
+                    ((gi + gj) & 1) << 7;
+        }
+    }
+    // the 3 + 12 + 12 values of each staged row
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int x = tid + q * 256;
+        if (x < HH * NL) {
+            const int ly = x / NL, k = x % NL;
+            const size_t gj = (size_t)wrap(j0 - H + ly, Ny);
+            rc[q] = k == 0 ? a.c1t[gj] : k == 1 ? a.c3t[gj] : k == 2 ? a.c4t[gj]
+                  : k < 15 ? a.c5t[gj * 12 + (k - 3)] : a.c5r[gj * 12 + (k - 15)];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        const int e = tid + q * 256;
+        if (e < E) {
+            s_chi[e] = rchi[q];
+            s_B[e] = rB[q];
+            s_m[e] = (uint8_t)rm[q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int x = tid + q * 256;
+        if (x < HH * NL) {
+            const int ly = x / NL, k = x % NL;
+            s_c[ly][k < 3 ? 2 * k + 1 : k + 3] = rc[q];
+            if (k < 3) s_c[ly][2 * k] = T(0);
+        }
     }
     __syncthreads();
     // Without an odd periodic wrap inside the block the colours alternate
     // along every staged row (colour = (p0 + lx + ly) % 2): each half-sweep
-    // then visits only the cells of its colour, two per thread fewer
-    // iterations; otherwise it walks the block and tests the colour bit.
+    // then visits only the cells of its colour; otherwise it walks the block
+    // and tests the colour bit.
     static_assert(W % 2 == 0, "pairs of cells along a row");
     const bool regular = ((Nx & 1) == 0 || (i0 - H >= 0 && i0 - H + W <= Nx)) &&
                          ((Ny & 1) == 0 || (j0 - H >= 0 && j0 - H + HH <= Ny));
     const int p0 = (i0 + j0) & 1;
     double acc = 0.0;
-#pragma unroll 1
+    // the update of cell (lx, ly) (sor_sweep's arithmetic): its new value, and
+    // the residual term r or res; land cells keep their value
+    auto upd = [&](int lx, int ly, T om, T& nv, T& rq) -> bool {
+        const int e = ly * W + lx;
+        const int m = s_m[e];
+        const T* const c = s_c[ly];
+        const T c1 = c[TC1 + ((m >> 4) & 1)], c2 = c[TC1 + ((m >> 5) & 1)], c3 = c[TC3 + ((m >> 1) & 1)],
+                c4 = c[TC4 + (m & 1)];
+        const int c5i = m & 15;   // ((ou(i+1,j) + ou) * 2 + ov(i,j+1)) * 2 + ov
+        const T c5 = c[TC5 + c5i];
+        const T x = s_chi[e];
+        const T lhs = (((c1 * s_chi[e + 1] + c2 * s_chi[e - 1]) + c3 * s_chi[e + W]) + c4 * s_chi[e - W]) + c5 * x;
+        const T r = lhs - s_B[e];
+        const T res = dvr(om * r, c5, c[TC5R + c5i]);
+        const bool oc = (m >> 6) & 1;
+        nv = oc ? x - res : x;
+        rq = a.residual_mode ? r : res;
+        return oc;
+    };
+    // tail half sweeps sum the residuals of the output cells
+    auto add = [&](int lx, int ly, T rq) {
+        if (lx >= H && lx < H + TX && ly >= H && ly < H + TY && i0 + lx - H < Nx && j0 + ly - H < Ny) {
+            const double q = (double)rq;
+            acc += q * q;
+        }
+    };
+    constexpr int NIM = (HH * (W / 2) + 255) / 256;   // cells of one colour per thread, at most
+#pragma unroll
     for (int hs = 0; hs < 2 * K; ++hs) {
         const int color = hs & 1;
         const int lo = hs + 1;
@@ -2151,37 +2232,42 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
             if (h >= a.nrelax) h = a.nrelax - 1;
             om = T(a.relax[h]);
         }
-        const int n = regular ? HH * (W / 2) : E;
-        for (int q = tid; q < n; q += 256) {
-            int lx, ly;
-            if (regular) {
-                ly = q / (W / 2);
-                lx = 2 * (q % (W / 2)) + ((color + p0 + ly) & 1);
-            } else {
-                lx = q % W;
-                ly = q / W;
+        if (regular) {
+            // rows lo .. HH-1-lo, (W - 2 lo) / 2 cells of the colour in each
+            const int RW2 = (W - 2 * lo) / 2, n = (HH - 2 * lo) * RW2;
+            T nv[NIM], rq[NIM];
+            int lxs[NIM], lys[NIM];
+            bool oc[NIM];
+            // slots past the region's cells (k * 256 >= n, known once the
+            // half sweeps are unrolled) are dropped; in the last one, threads
+            // past the last cell repeat it and drop the result, so every
+            // slot's reads are issued unconditionally
+#pragma unroll
+            for (int k = 0; k < NIM; ++k) {
+                oc[k] = false;
+                if (k * 256 >= n) continue;
+                const int q = tid + k * 256 < n ? tid + k * 256 : n - 1;
+                lys[k] = lo + q / RW2;
+                lxs[k] = lo + 2 * (q % RW2) + ((color + p0 + lys[k] + lo) & 1);
+                oc[k] = upd(lxs[k], lys[k], om, nv[k], rq[k]) && tid + k * 256 < n;
             }
-            if (lx < lo || lx > W - 1 - lo || ly < lo || ly > HH - 1 - lo) continue;
-            const int e = ly * W + lx;
-            const uint8_t m = s_m[e];
-            if ((m >> 7) != color || !(m & OC_E)) continue;
-            const int j = s_row[ly];
-            const uint8_t mip = s_m[e + 1], mjp = s_m[e + W];
-            const int ouip = (mip >> 1) & 1, ou = (m >> 1) & 1, ovjp = (mjp >> 2) & 1, ov = (m >> 2) & 1;
-            const T c1 = ouip ? a.c1t[j] : T(0);
-            const T c2 = ou ? a.c1t[j] : T(0);
-            const T c3 = ovjp ? a.c3t[j] : T(0);
-            const T c4 = ov ? a.c4t[j] : T(0);
-            const int c5i = ((j * 3 + ouip + ou) * 2 + ovjp) * 2 + ov;
-            const T c5 = a.c5t[c5i];
-            const T x = s_chi[e];
-            const T lhs = (((c1 * s_chi[e + 1] + c2 * s_chi[e - 1]) + c3 * s_chi[e + W]) + c4 * s_chi[e - W]) + c5 * x;
-            const T r = lhs - s_B[e];
-            const T res = dvr(om * r, c5, a.c5r[c5i]);
-            s_chi[e] = x - res;
-            if (tail && lx >= H && lx < H + TX && ly >= H && ly < H + TY && i0 + lx - H < Nx && j0 + ly - H < Ny) {
-                const double q = a.residual_mode ? (double)r : (double)res;
-                acc += q * q;
+#pragma unroll
+            for (int k = 0; k < NIM; ++k) {
+                if (k * 256 < n && oc[k]) {
+                    s_chi[lys[k] * W + lxs[k]] = nv[k];
+                    if (tail) add(lxs[k], lys[k], rq[k]);
+                }
+            }
+        } else {
+            for (int q = tid; q < E; q += 256) {
+                const int lx = q % W, ly = q / W;
+                if (lx < lo || lx > W - 1 - lo || ly < lo || ly > HH - 1 - lo) continue;
+                if ((s_m[q] >> 7) != color) continue;
+                T nv, rq;
+                if (upd(lx, ly, om, nv, rq)) {
+                    s_chi[q] = nv;
+                    if (tail) add(lx, ly, rq);
+                }
             }
         }
         __syncthreads();
