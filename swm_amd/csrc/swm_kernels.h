@@ -2007,6 +2007,13 @@ struct SorArgs {
     double eps, bnorm;
     long long* iters;
     double* norm_out;
+    // reference mode with two sweeps per launch (mid): the first sweep's chi of
+    // the output cells and its residual partials are kept as well, and the
+    // next launch's block 0 tests that sweep before the second one
+    int mid, prev_mid;
+    T* chi_mid;
+    double* partial_mid;
+    const double* prev_partial_mid;
 };
 
 // The convergence test of one sweep, by the 256 threads of one block: sum the
@@ -2127,10 +2134,15 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
     // a test has passed, later launches return at once and the converged chi
     // stays in the buffer that launch wrote (the host picks it by the sweep).
     __shared__ int conv;
-    if (a.prev_check && blockIdx.x == 0 && !*a.done &&
-        sor_test(a.prev_partial, (int)gridDim.x, a.prev_sweep, a.eps, a.bnorm, a.residual_mode ? 1 : 0, a.done,
-                 a.iters, a.norm_out, red, &conv))
-        return;
+    if (a.prev_check && blockIdx.x == 0 && !*a.done) {
+        const int mode = a.residual_mode ? 1 : 0;
+        if (a.prev_mid && sor_test(a.prev_partial_mid, (int)gridDim.x, a.prev_sweep - 1, a.eps, a.bnorm, mode, a.done,
+                                   a.iters, a.norm_out, red, &conv))
+            return;
+        if (sor_test(a.prev_partial, (int)gridDim.x, a.prev_sweep, a.eps, a.bnorm, mode, a.done, a.iters,
+                     a.norm_out, red, &conv))
+            return;
+    }
     if (*a.done) return;
     // every global load of the stage is issued before the first LDS store, so
     // the block waits for one memory latency, not one per cell it stages
@@ -2191,7 +2203,7 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
     const bool regular = ((Nx & 1) == 0 || (i0 - H >= 0 && i0 - H + W <= Nx)) &&
                          ((Ny & 1) == 0 || (j0 - H >= 0 && j0 - H + HH <= Ny));
     const int p0 = (i0 + j0) & 1;
-    double acc = 0.0;
+    double acc = 0.0, acc_mid = 0.0;
     // the update of cell (lx, ly) (sor_sweep's arithmetic): its new value, and
     // the residual term r or res; land cells keep their value
     auto upd = [&](int lx, int ly, T om, T& nv, T& rq) -> bool {
@@ -2211,11 +2223,17 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
         rq = a.residual_mode ? r : res;
         return oc;
     };
-    // tail half sweeps sum the residuals of the output cells
-    auto add = [&](int lx, int ly, T rq) {
+    // tested half sweeps sum the residuals of the output cells
+    auto add = [&](double& sum, int lx, int ly, T rq) {
         if (lx >= H && lx < H + TX && ly >= H && ly < H + TY && i0 + lx - H < Nx && j0 + ly - H < Ny) {
             const double q = (double)rq;
-            acc += q * q;
+            sum += q * q;
+        }
+    };
+    auto store_out = [&](T* out) {   // the output cells of the tile
+        for (int o = tid; o < TX * TY; o += 256) {
+            const int i = i0 + o % TX, j = j0 + o / TX;
+            if (i < Nx && j < Ny) out[(size_t)j * P + i] = s_chi[(H + o / TX) * W + H + o % TX];
         }
     };
     constexpr int NIM = (HH * (W / 2) + 255) / 256;   // cells of one colour per thread, at most
@@ -2224,6 +2242,7 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
         const int color = hs & 1;
         const int lo = hs + 1;
         const bool tail = a.accumulate && hs >= 2 * K - 2;
+        const bool first = K == 2 && a.mid && hs < 2;   // the first sweep, tested too
         // omega of this half-sweep: the reference's Chebyshev schedule (held
         // once stationary) or the fixed omega; a.sweep is the launch's first sweep
         T om = T(a.omega);
@@ -2255,7 +2274,8 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
             for (int k = 0; k < NIM; ++k) {
                 if (k * 256 < n && oc[k]) {
                     s_chi[lys[k] * W + lxs[k]] = nv[k];
-                    if (tail) add(lxs[k], lys[k], rq[k]);
+                    if (tail) add(acc, lxs[k], lys[k], rq[k]);
+                    if (first) add(acc_mid, lxs[k], lys[k], rq[k]);
                 }
             }
         } else {
@@ -2266,22 +2286,34 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
                 T nv, rq;
                 if (upd(lx, ly, om, nv, rq)) {
                     s_chi[q] = nv;
-                    if (tail) add(lx, ly, rq);
+                    if (tail) add(acc, lx, ly, rq);
+                    if (first) add(acc_mid, lx, ly, rq);
                 }
             }
         }
         __syncthreads();
+        if (K == 2 && hs == 1 && a.mid) {   // chi after the first sweep, before the second changes it
+            store_out(a.chi_mid);
+            __syncthreads();
+        }
     }
-    for (int o = tid; o < TX * TY; o += 256) {
-        const int i = i0 + o % TX, j = j0 + o / TX;
-        if (i < Nx && j < Ny) chi_out[(size_t)j * P + i] = s_chi[(H + o / TX) * W + H + o % TX];
-    }
+    store_out(chi_out);
     if (!a.accumulate) return;   // uniform
-    for (int s = 32; s >= 1; s >>= 1) acc += __shfl_xor(acc, s, 64);
-    const int lane = tid & 63, wv = tid >> 6;
-    if (lane == 0) red[wv] = acc;
-    __syncthreads();
-    if (tid == 0) a.partial[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+    // fixed-order block sum: wavefront butterfly, then the four wave sums in order
+    auto block_sum = [&](double v) -> double {
+        for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = v;
+        __syncthreads();
+        const double t = ((red[0] + red[1]) + red[2]) + red[3];
+        __syncthreads();
+        return t;
+    };
+    const double tot = block_sum(acc);
+    if (tid == 0) a.partial[blockIdx.x] = tot;
+    if (K == 2 && a.mid) {
+        const double tot_mid = block_sum(acc_mid);
+        if (tid == 0) a.partial_mid[blockIdx.x] = tot_mid;
+    }
 }
 
 // Sum the block partials in a fixed order (one block), test convergence.

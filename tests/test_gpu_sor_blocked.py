@@ -112,9 +112,9 @@ def test_blocked_stops_at_chosen_test_sweep(gpu_lib, every, stop):
 @pytest.mark.parametrize("blocked", ["1", "0"])
 def test_reference_mode_one_sweep_blocks(gpu_lib, monkeypatch, blocked):
     """main_ElSolv_SOR (Chebyshev omega per half-sweep, test after every sweep)
-    with one sweep per launch on LDS tiles, and with one colour per launch: the
-    oracle's chi bit for bit and its sweep count, on a grid of several tiles
-    with an odd row count."""
+    with blocked launches on LDS tiles (two sweeps per launch by default), and
+    with one colour per launch: the oracle's chi bit for bit and its sweep
+    count, on a grid of several tiles with an odd row count."""
     monkeypatch.setenv("SWM_GPU_SOR_BLOCK", blocked)
     s = basin(130, 75, 0.06, seed=13)
     o = oracle.Oracle(s)
@@ -125,5 +125,39 @@ def test_reference_mode_one_sweep_blocks(gpu_lib, monkeypatch, blocked):
     assert ok and rc_o == 0 and it_g == it_o
     assert same(chi_g, chi_o)
     assert abs(an_g - an_o) <= 1e-12 * an_o
+    g.close()
+    o.close()
+
+
+# Tolerances whose first passed test in the oracle's reference-mode solve of
+# the problem below is the given sweep; each lies inside the interval of
+# tolerances that stop there with a margin of at least 0.3 % on either side,
+# far above the rounding of a differently ordered sum of res^2.
+REF_STOPS = {2271: 0.01402977285267049, 2272: 0.013896860975592949,
+             2300: 0.01105465311498026, 2301: 0.010981280786951706}
+
+
+@pytest.mark.parametrize("halo", [-1, 2, 0])
+def test_reference_mode_stops_at_either_sweep_of_a_launch(gpu_lib, halo):
+    """main_ElSolv_SOR with two sweeps per launch (halo -1: each launch also
+    keeps its first sweep's chi and residual partials, and the next launch's
+    block 0 tests that sweep before the second), one sweep per launch (halo 2)
+    and one colour per launch (halo 0).  Sweeps 2271 and 2272 are the two
+    sweeps of the last launch of a host batch (tested by the one-block checks
+    that close the batch), 2301 and 2300 the first and second sweep of launches
+    inside a batch (tested by the next launch): for each the oracle's sweep
+    count and chi, bit for bit, and its norm to rounding."""
+    s = basin(130, 75, 0.06, seed=13)
+    o = oracle.Oracle(s)
+    B = rhs(o, s, 8)
+    g = gpu.SwmGpu(s)
+    g.set_sor_halo(halo)
+    for k, eps in REF_STOPS.items():
+        chi_o, it_o, an_o, rc_o = o.sor(B, np.zeros(s.shape), epsilon=eps, first_guess=True)
+        assert rc_o == 0 and it_o == k
+        chi_g, it_g, an_g, ok = g.sor(B, np.zeros(s.shape), epsilon=eps, first_guess=True)
+        assert ok and it_g == k, (halo, k, it_g)
+        assert same(chi_g, chi_o), (halo, k)
+        assert abs(an_g - an_o) <= 1e-12 * an_o
     g.close()
     o.close()
