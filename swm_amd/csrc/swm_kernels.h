@@ -262,11 +262,19 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 #define OU(m) (((m) >> 1) & 1)
 #define OH(m) ((m) & 1)
 
+// a * b + c.  FP64: the product and the sum rounded separately, as written
+// (the file is compiled without contraction); IEEE addition is commutative, so
+// a sum of products written as nested fm's, p1 + p2 + ... = fm(.., fm(b2, c2,
+// p1)), keeps the bits of the sum in the reference's order.  FP32: one fused
+// multiply-add (FP32 storage carries an error bar, not bitwise parity).
+__device__ __forceinline__ double fm(double a, double b, double c) { return a * b + c; }
+__device__ __forceinline__ float fm(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 // interp2 with *_noland weights of D between two eta points (eta2u / eta2v)
 template <typename T>
 __device__ __forceinline__ T interp_noland2(T D1, T D2, int m1, int m2) {
     const T w = noland_w<T>(m1 + m2);
-    return (m1 ? w : T(0)) * D1 + (m2 ? w : T(0)) * D2;
+    return fm(m2 ? w : T(0), D2, (m1 ? w : T(0)) * D1);
 }
 
 // Arithmetic of the step kernels.  FP64 evaluates every division and
@@ -816,8 +824,8 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             if (OE(m)) {
                 if (FN) {
                     const T ue = s_u[ke], uc = s_u[e], vn = s_v[kn], vc = s_v[e];
-                    Ed = ((T(0.5) * (ue * ue) + T(0.5) * (uc * uc)) + (T(0.5) * (vn * vn) + T(0.5) * (vc * vc))) / T(2) +
-                         a.G * s_eta[e];
+                    Ed = fm(a.G, s_eta[e],
+                            (fm(T(0.5), uc * uc, T(0.5) * (ue * ue)) + fm(T(0.5), vc * vc, T(0.5) * (vn * vn))) / T(2));
                 } else if (VAR == VAR_MS) {   // g*eta + uU + vV
                     const int gj = wrap(j0 + ly - 1, Ny);
                     const int gi = wrap(i0 + lx - 1, Nx), gip = gi + 1 == Nx ? 0 : gi + 1;
@@ -837,7 +845,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             T pll = T(0);
             if (a.mix && OE(m)) {
                 const T* c = &lds.pllc[ly][(OV(M(kn)) * 2 + OV(m)) * 3];
-                pll = s_D[e] * ((c[0] * (s_u[ke] - s_u[e]) + c[1] * s_v[kn]) + c[2] * s_v[e]);
+                pll = s_D[e] * fm(c[2], s_v[e], fm(c[1], s_v[kn], c[0] * (s_u[ke] - s_u[e])));
             }
             s_pll[e] = pll;
         }
@@ -850,8 +858,8 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             const int kw = e - 1, ks = e - EW, ksw = e - EW - 1;
             const int m1 = OE(m), m2 = OE(M(ksw)), m3 = OE(M(kw)), m4 = OE(M(ks));
             const T w = noland_w<T>(m1 + m2 + m3 + m4);
-            const T Dh = (((m1 ? w : T(0)) * s_D[e] + (m2 ? w : T(0)) * s_D[ksw]) + (m3 ? w : T(0)) * s_D[kw]) +
-                         (m4 ? w : T(0)) * s_D[ks];
+            const T Dh = fm(m4 ? w : T(0), s_D[ks],
+                            fm(m3 ? w : T(0), s_D[kw], fm(m2 ? w : T(0), s_D[ksw], (m1 ? w : T(0)) * s_D[e])));
             T zeta = T(0);
             if (FN || VAR == VAR_MS) {
                 T za_ = T(0), zb = T(0);
@@ -859,7 +867,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                     za_ = lds.za[ly][OH(m) ? 0 : 1] * (s_v[e] - s_v[kw]);
                 if (OU(m) + OU(M(ks)) != 0) {
                     const T bcH = OH(m) ? T(1) : a.lbc;
-                    zb = dvr(bcH * (lds.cu[ly + 1] * s_u[e] - lds.cu[ly] * s_u[ks]), a.AdT, a.rAdT);
+                    zb = dvr(bcH * fm(-lds.cu[ly], s_u[ks], lds.cu[ly + 1] * s_u[e]), a.AdT, a.rAdT);
                 }
                 zeta = za_ - dvr(zb, lds.cH[ly], lds.rcH[ly]);
             }
@@ -876,7 +884,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             T plt = T(0);
             if (a.mix && OH(m)) {
                 const T* c = &lds.pltc[ly][(OU(m) * 2 + OU(M(ks))) * 3];
-                plt = Dh * ((c[0] * (s_v[e] - s_v[kw]) + c[1] * s_u[e]) + c[2] * s_u[ks]);
+                plt = Dh * fm(c[2], s_u[ks], fm(c[1], s_u[e], c[0] * (s_v[e] - s_v[kw])));
             }
             s_plt[e] = plt;
         }
@@ -894,8 +902,8 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             // Dh = interp4(D, eta2H_noland): points (i,j), (i-1,j-1), (i-1,j), (i,j-1)
             const int m1 = OE(m), m2 = OE(M(ksw)), m3 = OE(M(kw)), m4 = OE(M(ks));
             const T w = noland_w<T>(m1 + m2 + m3 + m4);
-            const T Dh = (((m1 ? w : T(0)) * s_D[e] + (m2 ? w : T(0)) * s_D[ksw]) + (m3 ? w : T(0)) * s_D[kw]) +
-                         (m4 ? w : T(0)) * s_D[ks];
+            const T Dh = fm(m4 ? w : T(0), s_D[ks],
+                            fm(m3 ? w : T(0), s_D[kw], fm(m2 ? w : T(0), s_D[ksw], (m1 ? w : T(0)) * s_D[e])));
             // computeRelVort -> vorticityFromVelocities2D (src/calc_lib.f90:663-678,
             // pder_zonal3D 892-926, pder_meridional3D 965-998)
             T zeta = T(0);
@@ -906,7 +914,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                     za_ = lds.za[ly][OH(m) ? 0 : 1] * (s_v[e] - s_v[kw]);
                 if (OU(m) + OU(M(ks)) != 0) {
                     const T bcH = OH(m) ? T(1) : a.lbc;
-                    zb = dvr(bcH * (lds.cu[ly + 1] * s_u[e] - lds.cu[ly] * s_u[ks]), a.AdT, a.rAdT);
+                    zb = dvr(bcH * fm(-lds.cu[ly], s_u[ks], lds.cu[ly + 1] * s_u[e]), a.AdT, a.rAdT);
                 }
                 zeta = za_ - dvr(zb, lds.cH[ly], lds.rcH[ly]);
             }
@@ -925,7 +933,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             T plt = T(0);
             if (a.mix && OH(m)) {
                 const T* c = &lds.pltc[ly][(OU(m) * 2 + OU(M(ks))) * 3];
-                plt = Dh * ((c[0] * (s_v[e] - s_v[kw]) + c[1] * s_u[e]) + c[2] * s_u[ks]);
+                plt = Dh * fm(c[2], s_u[ks], fm(c[1], s_u[e], c[0] * (s_v[e] - s_v[kw])));
             }
             s_plt[e] = plt;
             if (interior) { gst(a.diag[1], doff, Dh); gst(a.diag[4], doff, zeta); gst(a.diag[5], doff, Pot); }
@@ -950,8 +958,8 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             if (OE(m)) {
                 if (FN) {
                     const T ue = s_u[ke], uc = s_u[e], vn = s_v[kn], vc = s_v[e];
-                    Ed = ((T(0.5) * (ue * ue) + T(0.5) * (uc * uc)) + (T(0.5) * (vn * vn) + T(0.5) * (vc * vc))) / T(2) +
-                         a.G * eta_e;
+                    Ed = fm(a.G, eta_e,
+                            (fm(T(0.5), uc * uc, T(0.5) * (ue * ue)) + fm(T(0.5), vc * vc, T(0.5) * (vn * vn))) / T(2));
                 } else if (VAR == VAR_MS) {   // g*eta + uU + vV
                     const int gi = wrap(i0 + lx - 1, Nx), gip = gi + 1 == Nx ? 0 : gi + 1;
                     const int gjp = gj + 1 == Ny ? 0 : gj + 1;
@@ -971,7 +979,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             T pll = T(0);
             if (a.mix && OE(m)) {
                 const T* c = &lds.pllc[ly][(OV(M(kn)) * 2 + OV(m)) * 3];
-                pll = s_D[e] * ((c[0] * (s_u[ke] - s_u[e]) + c[1] * s_v[kn]) + c[2] * s_v[e]);
+                pll = s_D[e] * fm(c[2], s_v[e], fm(c[1], s_v[kn], c[0] * (s_u[ke] - s_u[e])));
             }
             s_pll[e] = pll;
             if (interior) { gst(a.diag[6], doff, Ed); gst(a.diag[0], doff, s_D[e]); }
@@ -1071,7 +1079,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
         const int ipx = i + 1 == Nx ? 0 : i + 1, imx = i == 0 ? Nx - 1 : i - 1, jmx = j == 0 ? Ny - 1 : j - 1;
         if (oe) {
             const T t1 = dvr(-(MUe - MUk), a.dL, a.rdL);
-            const T t2 = dvr(MVn * rv(RT_CV, 1) - MVk * rv(RT_CV, 0), a.dT, a.rdT);
+            const T t2 = dvr(fm(-MVk, rv(RT_CV, 0), MVn * rv(RT_CV, 1)), a.dT, a.rdT);
             T tt = t1 - t2;
             if constexpr (VAR == VAR_MS) {
                 // src/swm_timestep_module.f90:359-365.  The first interpolation is
@@ -1094,8 +1102,8 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             Ge = dvr(tt, rv(RT_ACE, 0), rv(RT_RACE, 0)) + Fe;
         }
         if (ou) {
-            const T P2 = T(0.5) * s_Pot[kn] + T(0.5) * s_Pot[k];
-            const T M4 = ((T(0.25) * MVn + T(0.25) * MVw) + T(0.25) * MVnw) + T(0.25) * MVk;
+            const T P2 = fm(T(0.5), s_Pot[k], T(0.5) * s_Pot[kn]);
+            const T M4 = fm(T(0.25), MVk, fm(T(0.25), MVnw, fm(T(0.25), MVw, T(0.25) * MVn)));
             T g = P2 * M4;
             if constexpr (VAR == VAR_MS) {   // + interp2(zeta, H2u) * interp4(v_bs, v2u)
                 const T Z2 = T(0.5) * lds.zt[kn] + T(0.5) * lds.zt[k];
@@ -1117,14 +1125,14 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                 const int ot = (oh + OH(M(kn))) < 1 ? 1 : (oh + OH(M(kn)));
                 const T c0 = rvs(ot == 1, RT_LMU, RT_LMU + 3), c1 = rvs(ot == 1, RT_LMU + 1, RT_LMU + 4),
                         c2 = rvs(ot == 1, RT_LMU + 2, RT_LMU + 5);
-                Lu = dv((c0 * (s_pll[k] - s_pll[kw]) + c1 * s_plt[kn]) + c2 * s_plt[k], Du);
+                Lu = dv(fm(c2, s_plt[k], fm(c1, s_plt[kn], c0 * (s_pll[k] - s_pll[kw]))), Du);
                 g = g + Lu;
             }
             Gu = g + Fx;
         }
         if (ov) {
-            const T P2 = T(0.5) * s_Pot[ke] + T(0.5) * s_Pot[k];
-            const T M4 = ((T(0.25) * MUe + T(0.25) * MUs) + T(0.25) * MUk) + T(0.25) * MUse;
+            const T P2 = fm(T(0.5), s_Pot[k], T(0.5) * s_Pot[ke]);
+            const T M4 = fm(T(0.25), MUse, fm(T(0.25), MUk, fm(T(0.25), MUs, T(0.25) * MUe)));
             T g = -(P2 * M4);
             if constexpr (VAR == VAR_MS) {   // - interp2(zeta, H2v) * interp4(u_bs, u2v)
                 const T Z2 = T(0.5) * lds.zt[ke] + T(0.5) * lds.zt[k];
@@ -1146,7 +1154,7 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                 const int ot = (oe + OE(M(ks))) < 1 ? 1 : (oe + OE(M(ks)));
                 const T c0 = rvs(ot == 1, RT_LMV, RT_LMV + 3), c1 = rvs(ot == 1, RT_LMV + 1, RT_LMV + 4),
                         c2 = rvs(ot == 1, RT_LMV + 2, RT_LMV + 5);
-                Lv = dv((c0 * (s_plt[ke] - s_plt[k]) + c1 * s_pll[k]) + c2 * s_pll[ks], Dv);
+                Lv = dv(fm(c2, s_pll[ks], fm(c1, s_pll[k], c0 * (s_plt[ke] - s_plt[k]))), Dv);
                 g = g + Lv;
             }
             Gv = g + Fy;
@@ -1258,10 +1266,9 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
                 const T c10 = ou ? -rv(RT_T9, 0) : T(-0.0);
                 const T c11 = ovn ? rv(RT_T11, 0) : T(0.0);
                 const T c12 = ov ? -rv(RT_T12, 0) : T(-0.0);
-                diff = ((((kap * hue) * (Ce - C)) * c9 + ((kap * Du) * (C - Cw)) * c10) +
-                        ((kap * hvn) * (Cn - C)) * c11) + ((kap * Dv) * (C - Cs)) * c12;
+                diff = fm((kap * Dv) * (C - Cs), c12,
+                          fm((kap * hvn) * (Cn - C), c11, fm((kap * Du) * (C - Cw), c10, ((kap * hue) * (Ce - C)) * c9)));
             }
-            const T forcing = C * Fe;
             const T c1 = ta(true, oue & oee, oee + oe, RT_TA);
             const T c2 = ta(false, ou & oew, oew + oe, RT_TA);
             const T c3 = ta(true, ovn & oen, oen + oe, RT_TB);
@@ -1270,16 +1277,17 @@ __device__ __forceinline__ void tile_compute(const Dyn<T>& d, const int t, L& ld
             const T c6 = ta(false, ou & oe, oew + oe, RT_TA);
             const T c7 = ta(true, ovn & oe, oen + oe, RT_TB);
             const T c8 = ta(false, ov & oe, oes + oe, RT_TC);
-            T g = (CHe * ume) * c1 + (CHw * um) * c2;
-            g = g + (CHn * vmn) * c3;
-            g = g + (CHs * vm) * c4;
-            g = g + (CH * ume) * c5;
-            g = g + (CH * um) * c6;
-            g = g + (CH * vmn) * c7;
-            g = g + (CH * vm) * c8;
+            T g = fm(CHw * um, c2, (CHe * ume) * c1);
+            g = fm(CHn * vmn, c3, g);
+            g = fm(CHs * vm, c4, g);
+            g = fm(CH * ume, c5, g);
+            g = fm(CH * um, c6, g);
+            g = fm(CH * vmn, c7, g);
+            g = fm(CH * vm, c8, g);
             g = g + diff;
-            g = g - (fl(a.gC[n], RT_TRC + RT_PER_TRC * n + 2, off, oe) * s_D[k]) * (C - fl(a.C0[n], RT_TRC + RT_PER_TRC * n + 4, off, oe));
-            g = g + forcing;
+            g = fm(-(fl(a.gC[n], RT_TRC + RT_PER_TRC * n + 2, off, oe) * s_D[k]),
+                   C - fl(a.C0[n], RT_TRC + RT_PER_TRC * n + 4, off, oe), g);
+            g = fm(C, Fe, g);   // + C * Fe
             T chn;
             if constexpr (sizeof(T) == 4) {
                 T inc = d.dtc1 * pt[q][n][0];
