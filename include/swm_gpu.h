@@ -230,10 +230,10 @@ int swm_gpu_avg_read(swm_ctx *c, int32_t avg_id, double *host_out, double *weigh
  * 2K cells for K sweeps per launch (2, 4 or 8; K must divide the cadence),
  * 0 for one launch per colour, -1 (default) for the largest of 4, 2, 1 sweeps
  * that divides the cadence.  The reference mode (test after every sweep) uses
- * K = 2 for halo -1, 4 or 8 (each launch also keeps its first sweep's chi and
- * residual, so either sweep can be the one that passes), K = 1 for halo 2 and
- * one launch per colour for halo 0.  Every halo gives the same chi bit for
- * bit and, in the reference mode, the same sweep count. */
+ * K = halo / 2 sweeps per launch, 4 for halo -1 (each launch also keeps the
+ * chi and residual of its earlier sweeps, so any of its sweeps can be the one
+ * that passes), and one launch per colour for halo 0.  Every halo gives the
+ * same chi bit for bit and, in the reference mode, the same sweep count. */
 int swm_gpu_kernel_shape_count(void);
 int swm_gpu_set_kernel_shape(swm_ctx *c, int32_t shape);
 int swm_gpu_tune(swm_ctx *c, int32_t steps, int32_t *best_shape, double *times_ms);

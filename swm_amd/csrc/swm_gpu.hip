@@ -251,8 +251,8 @@ struct swm_ctx {
     double* sor_norm = nullptr;
     void* sor_buf[2][2] = {};    // [fp32][chi|B]
     void* sor_alt[2] = {};       // second chi buffer of the blocked sweeps (ping-pong)
-    void* sor_mid[2][2] = {};    // reference mode, two sweeps per launch: chi after the first, by launch parity
-    double* sor_partial_mid = nullptr;   // and its partial sums (two sets by launch parity)
+    void* sor_mid[2][3][2] = {}; // reference mode, K sweeps per launch: chi after each of the first K - 1, by launch parity
+    double* sor_partial_mid = nullptr;   // and their partial sums (3 sweeps x two sets by launch parity)
     void* sor_ftab[2][5] = {};   // coefficient tables in the sweep precision
     void* dza = nullptr;
     // elliptic-solver build (SWM_ELLIPTIC_SOLVER): the reference-mode velocity
@@ -1602,9 +1602,9 @@ int sor_prepare(swm_ctx* c, bool f32, int nblk) {
 
 // Fixed-omega sweeps l .. l+K-1 in one blocked launch (sor_block) from chi_in
 // into chi_out, then the convergence test if sweep l+K-1 is a test sweep.
-// With mid (reference mode, K = 2) the launch also keeps the first sweep's chi
-// in sor_mid[.][parity] and its partial sums in sor_partial_mid, and block 0
-// of the next launch tests that sweep before the second.
+// With mid (reference mode) the launch also keeps chi after each of its first
+// K - 1 sweeps in sor_mid[.][sweep][parity] and their partial sums in
+// sor_partial_mid, and block 0 of the next launch tests the sweeps in order.
 template <typename T, int K, int TY>
 void sor_block_launch(swm_ctx* c, const void* chi_in, void* chi_out, void* dB, void* const* tabs, double omega,
                       double tol, double bnorm, int64_t l, bool check, bool ref_mode, int parity, int64_t prev_sweep,
@@ -1635,9 +1635,11 @@ void sor_block_launch(swm_ctx* c, const void* chi_in, void* chi_out, void* dB, v
         const int w = std::is_same<T, float>::value ? 1 : 0;
         a.mid = 1;
         a.prev_mid = a.prev_check;
-        a.chi_mid = (T*)c->sor_mid[w][parity];
-        a.partial_mid = c->sor_partial_mid + (size_t)parity * nb;
-        a.prev_partial_mid = c->sor_partial_mid + (size_t)(parity ^ 1) * nb;
+        for (int q = 0; q < K - 1; ++q) {
+            a.chi_mid[q] = (T*)c->sor_mid[w][q][parity];
+            a.partial_mid[q] = c->sor_partial_mid + (size_t)(2 * q + parity) * nb;
+            a.prev_partial_mid[q] = c->sor_partial_mid + (size_t)(2 * q + (parity ^ 1)) * nb;
+        }
     }
     hipLaunchKernelGGL((swm::sor_block<T, TX, TY, K>), dim3(nb), dim3(256), 0, c->stream, a, (const T*)chi_in,
                        (T*)chi_out, ntx);
@@ -1661,25 +1663,26 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
     // sweeps per launch K: a set halo (2K cells), else the largest of 4, 2, 1
     // that divides the test cadence; the reference mode tests every sweep (K <= 1)
     int K = c->sor_halo >= 0 ? c->sor_halo / 2 : (every % 4 == 0 ? 4 : (every % 2 == 0 ? 2 : 1));
-    // the reference mode tests every sweep: two sweeps per launch keep the
-    // first sweep's chi and residual too (mid), unless the halo asks for one
-    if (ref_mode) K = c->sor_halo < 0 ? 2 : std::min(2, c->sor_halo / 2);
+    // the reference mode tests every sweep: a launch of K > 1 sweeps keeps the
+    // chi and residual of its first K - 1 sweeps too (mid); K from the halo,
+    // 4 by default
+    if (ref_mode) K = c->sor_halo < 0 ? 4 : c->sor_halo / 2;
     if (!c->sor_blocked) K = 0;
-    const bool mid = ref_mode && K == 2;
+    const bool mid = ref_mode && K > 1;
     if (!ref_mode && K > 1 && every % K) return fail(c, "the SOR halo's sweeps per launch must divide the test cadence", 1);
     const int rows = (K == 4) ? 16 : c->sor_rows;   // 64x32 tiles with a 4-sweep halo exceed 64 KiB of LDS
     if (mid) {
         const int w = std::is_same<T, float>::value ? 1 : 0;
         const size_t bytes = (size_t)c->P * c->Ny * sizeof(T);
         if (!c->sor_partial_mid) {
-            const size_t nbm = 2 * (size_t)((c->Nx + 63) / 64) * ((c->Ny + 15) / 16);
+            const size_t nbm = 6 * (size_t)((c->Nx + 63) / 64) * ((c->Ny + 15) / 16);
             c->sor_partial_mid = (double*)dalloc(c, sizeof(double) * nbm);
+            if (!c->sor_partial_mid) return fail(c, "out of device memory", 3);
         }
-        if (!c->sor_mid[w][0]) {
-            c->sor_mid[w][0] = dalloc(c, bytes);
-            c->sor_mid[w][1] = dalloc(c, bytes);
-        }
-        if (!c->sor_partial_mid || !c->sor_mid[w][0] || !c->sor_mid[w][1]) return fail(c, "out of device memory", 3);
+        for (int q = 0; q < K - 1; ++q)
+            for (int par = 0; par < 2; ++par)
+                if (!c->sor_mid[w][q][par] && !(c->sor_mid[w][q][par] = dalloc(c, bytes)))
+                    return fail(c, "out of device memory", 3);
     }
     if (K) {
         // Launch n does sweeps n*K+1 .. (n+1)*K from one buffer into the other
@@ -1700,7 +1703,7 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
             for (; l + K - 1 <= lend; l += K, ++n) {
                 const bool check = ((l + K - 1) % every) == 0 || l + K - 1 == max_sweeps;
                 const int par = (int)(n & 1);
-                if (K == 4) sor_block_launch<T, 4, 16>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending);
+                if (K == 4) sor_block_launch<T, 4, 16>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending, mid);
                 else if (K == 2 && rows == 32) sor_block_launch<T, 2, 32>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending, mid);
                 else if (K == 2) sor_block_launch<T, 2, 16>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending, mid);
                 else if (rows == 32) sor_block_launch<T, 1, 32>(c, cur, alt, dB, tabs, omega, tol, bnorm, l, check, ref_mode, par, pending);
@@ -1708,11 +1711,13 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
                 pending = check ? l + K - 1 : 0;
                 std::swap(cur, alt);
             }
-            if (pending) {   // the last launch's test(s): its first sweep first (mid), then its last
+            if (pending) {   // the last launch's test(s): its earlier sweeps in order (mid), then its last
                 if (mid)
-                    hipLaunchKernelGGL(swm::sor_check, dim3(1), dim3(256), 0, c->stream,
-                                       (const double*)c->sor_partial_mid + (size_t)((n - 1) & 1) * nb, nb,
-                                       (int)pending - 1, tol, bnorm, 0, c->sor_done, c->sor_iters, c->sor_norm);
+                    for (int q = 0; q < K - 1; ++q)
+                        hipLaunchKernelGGL(swm::sor_check, dim3(1), dim3(256), 0, c->stream,
+                                           (const double*)c->sor_partial_mid + (size_t)(2 * q + ((n - 1) & 1)) * nb,
+                                           nb, (int)(pending - (K - 1) + q), tol, bnorm, 0, c->sor_done,
+                                           c->sor_iters, c->sor_norm);
                 hipLaunchKernelGGL(swm::sor_check, dim3(1), dim3(256), 0, c->stream,
                                    (const double*)c->sor_partial + (size_t)((n - 1) & 1) * nb, nb, (int)pending, tol,
                                    bnorm, ref_mode ? 0 : 1, c->sor_done, c->sor_iters, c->sor_norm);
@@ -1724,7 +1729,8 @@ int sor_sweeps(swm_ctx* c, void* dchi, void* dB, void* const* tabs, bool ref_mod
             long long it = 0;
             HIPCHK(c, copy_sync(c, &it, c->sor_iters, sizeof(it), hipMemcpyDeviceToHost));
             const int64_t nconv = (it - 1) / K;   // launch n did sweeps n*K+1 .. (n+1)*K
-            if (mid && (it - 1) % 2 == 0) cur = c->sor_mid[w][nconv & 1];   // its first sweep passed
+            const int64_t q = (it - 1) % K;       // the passed sweep's place in it
+            if (mid && q < K - 1) cur = c->sor_mid[w][q][nconv & 1];
             else cur = (nconv & 1) ? dchi : c->sor_alt[w];
         }
         if (cur != dchi)   // the result in the caller's buffer

@@ -2015,13 +2015,13 @@ struct SorArgs {
     double eps, bnorm;
     long long* iters;
     double* norm_out;
-    // reference mode with two sweeps per launch (mid): the first sweep's chi of
-    // the output cells and its residual partials are kept as well, and the
-    // next launch's block 0 tests that sweep before the second one
+    // reference mode with K sweeps per launch (mid): the chi of the output
+    // cells after each of the first K - 1 sweeps and their residual partials
+    // are kept as well, and the next launch's block 0 tests the sweeps in order
     int mid, prev_mid;
-    T* chi_mid;
-    double* partial_mid;
-    const double* prev_partial_mid;
+    T* chi_mid[3];
+    double* partial_mid[3];
+    const double* prev_partial_mid[3];
 };
 
 // The convergence test of one sweep, by the 256 threads of one block: sum the
@@ -2144,9 +2144,11 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
     __shared__ int conv;
     if (a.prev_check && blockIdx.x == 0 && !*a.done) {
         const int mode = a.residual_mode ? 1 : 0;
-        if (a.prev_mid && sor_test(a.prev_partial_mid, (int)gridDim.x, a.prev_sweep - 1, a.eps, a.bnorm, mode, a.done,
-                                   a.iters, a.norm_out, red, &conv))
-            return;
+        if (a.prev_mid)
+            for (int sw = 0; sw < K - 1; ++sw)
+                if (sor_test(a.prev_partial_mid[sw], (int)gridDim.x, a.prev_sweep - (K - 1) + sw, a.eps, a.bnorm, mode,
+                             a.done, a.iters, a.norm_out, red, &conv))
+                    return;
         if (sor_test(a.prev_partial, (int)gridDim.x, a.prev_sweep, a.eps, a.bnorm, mode, a.done, a.iters,
                      a.norm_out, red, &conv))
             return;
@@ -2211,7 +2213,7 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
     const bool regular = ((Nx & 1) == 0 || (i0 - H >= 0 && i0 - H + W <= Nx)) &&
                          ((Ny & 1) == 0 || (j0 - H >= 0 && j0 - H + HH <= Ny));
     const int p0 = (i0 + j0) & 1;
-    double acc = 0.0, acc_mid = 0.0;
+    double acc = 0.0, acc_mid[3] = {0.0, 0.0, 0.0};
     // the update of cell (lx, ly) (sor_sweep's arithmetic): its new value, and
     // the residual term r or res; land cells keep their value
     auto upd = [&](int lx, int ly, T om, T& nv, T& rq) -> bool {
@@ -2250,7 +2252,8 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
         const int color = hs & 1;
         const int lo = hs + 1;
         const bool tail = a.accumulate && hs >= 2 * K - 2;
-        const bool first = K == 2 && a.mid && hs < 2;   // the first sweep, tested too
+        const int sw = hs / 2;                          // the sweep of this half sweep
+        const bool inner = a.mid && sw < K - 1;         // an earlier sweep, tested too (mid)
         // omega of this half-sweep: the reference's Chebyshev schedule (held
         // once stationary) or the fixed omega; a.sweep is the launch's first sweep
         T om = T(a.omega);
@@ -2283,7 +2286,7 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
                 if (k * 256 < n && oc[k]) {
                     s_chi[lys[k] * W + lxs[k]] = nv[k];
                     if (tail) add(acc, lxs[k], lys[k], rq[k]);
-                    if (first) add(acc_mid, lxs[k], lys[k], rq[k]);
+                    if (inner) add(acc_mid[sw < 3 ? sw : 0], lxs[k], lys[k], rq[k]);
                 }
             }
         } else {
@@ -2295,13 +2298,13 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
                 if (upd(lx, ly, om, nv, rq)) {
                     s_chi[q] = nv;
                     if (tail) add(acc, lx, ly, rq);
-                    if (first) add(acc_mid, lx, ly, rq);
+                    if (inner) add(acc_mid[sw < 3 ? sw : 0], lx, ly, rq);
                 }
             }
         }
         __syncthreads();
-        if (K == 2 && hs == 1 && a.mid) {   // chi after the first sweep, before the second changes it
-            store_out(a.chi_mid);
+        if ((hs & 1) && inner) {   // chi after sweep sw, before the next sweep changes it
+            store_out(a.chi_mid[sw < 3 ? sw : 0]);
             __syncthreads();
         }
     }
@@ -2318,9 +2321,12 @@ __global__ void __launch_bounds__(256) sor_block(const SorArgs<T> a, const T* __
     };
     const double tot = block_sum(acc);
     if (tid == 0) a.partial[blockIdx.x] = tot;
-    if (K == 2 && a.mid) {
-        const double tot_mid = block_sum(acc_mid);
-        if (tid == 0) a.partial_mid[blockIdx.x] = tot_mid;
+    if (a.mid) {
+#pragma unroll
+        for (int q = 0; q < K - 1 && q < 3; ++q) {
+            const double tot_mid = block_sum(acc_mid[q]);
+            if (tid == 0) a.partial_mid[q][blockIdx.x] = tot_mid;
+        }
     }
 }
 

@@ -133,24 +133,26 @@ def test_reference_mode_one_sweep_blocks(gpu_lib, monkeypatch, blocked):
 # the problem below is the given sweep; each lies inside the interval of
 # tolerances that stop there with a margin of at least 0.3 % on either side,
 # far above the rounding of a differently ordered sum of res^2.
-REF_STOPS = {2271: 0.01402977285267049, 2272: 0.013896860975592949,
-             2300: 0.01105465311498026, 2301: 0.010981280786951706}
+REF_STOPS = {2270: 0.014168205745143653, 2271: 0.01402977285267049, 2272: 0.013896860975592949,
+             2300: 0.01105465311498026, 2301: 0.010981280786951706, 2302: 0.010906213987353636}
 
 
-@pytest.mark.parametrize("halo", [-1, 2, 0])
-def test_reference_mode_stops_at_either_sweep_of_a_launch(gpu_lib, halo):
-    """main_ElSolv_SOR with two sweeps per launch (halo -1: each launch also
-    keeps its first sweep's chi and residual partials, and the next launch's
-    block 0 tests that sweep before the second), one sweep per launch (halo 2)
-    and one colour per launch (halo 0).  Sweeps 2271 and 2272 are the two
-    sweeps of the last launch of a host batch (tested by the one-block checks
-    that close the batch), 2301 and 2300 the first and second sweep of launches
-    inside a batch (tested by the next launch): for each the oracle's sweep
-    count and chi, bit for bit, and its norm to rounding."""
+@pytest.mark.parametrize("halo", [-1, 8, 4, 2, 0])
+def test_reference_mode_stops_at_any_sweep_of_a_launch(gpu_lib, halo):
+    """main_ElSolv_SOR with four sweeps per launch (halo -1 or 8) or two (halo
+    4) — each launch also keeps the chi and residual partials of its earlier
+    sweeps, and the next launch's block 0 tests the sweeps in order — with one
+    sweep per launch (halo 2) and one colour per launch (halo 0).  Host
+    batches end at sweep 2272, so 2270..2272 lie in the last launch of a batch
+    (tested by the one-block checks that close it) and 2300..2302 inside one
+    (tested by the next launch); with four sweeps per launch they are the 2nd,
+    3rd, 4th and 4th, 1st, 2nd sweep of their launch.  For each, the oracle's
+    sweep count and chi, bit for bit, and its norm to rounding."""
     s = basin(130, 75, 0.06, seed=13)
     o = oracle.Oracle(s)
     B = rhs(o, s, 8)
     g = gpu.SwmGpu(s)
+    g.set_sor_check_every(16)   # fixed-omega cadence that the 8-cell halo's four sweeps divide
     g.set_sor_halo(halo)
     for k, eps in REF_STOPS.items():
         chi_o, it_o, an_o, rc_o = o.sor(B, np.zeros(s.shape), epsilon=eps, first_guess=True)
