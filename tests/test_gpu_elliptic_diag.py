@@ -67,3 +67,29 @@ def test_fp32_nondivergent_flow_close_to_fp64(gpu_lib):
         assert np.abs(x - y).max() <= 1e-3 * np.abs(x).max(), name
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("precision", [gpu.FP64, gpu.FP32])
+def test_solve_is_the_same_at_every_sweeps_per_launch(gpu_lib, precision):
+    """The elliptic build's per-step solve (reference mode, test after every
+    sweep) with four, two and one sweep(s) per launch (halo 8, 4, 2: launches
+    of several sweeps keep and test each earlier sweep's chi and residual) and
+    one colour per launch (halo 0): the same sweep count and the same CHI bit
+    for bit at every step, in the context's precision (FP32 sweeps for an FP32
+    context)."""
+    s = setup()
+    runs = {}
+    for halo in (8, 4, 2, 0):
+        g = gpu.SwmGpu(s, precision).init_from_setup()
+        g.set_sor_check_every(16)        # fixed-omega cadence that the 8-cell halo's four sweeps divide
+        g.set_sor_halo(halo)
+        runs[halo] = []
+        for itt in (1, 2, 7):
+            g.step(itt - g.itt)
+            chi = g.get("CHI")
+            runs[halo].append((g.elliptic_stats()[0], chi))
+        g.close()
+    for halo in (4, 2, 0):
+        for (it_a, chi_a), (it_b, chi_b) in zip(runs[8], runs[halo]):
+            assert it_a == it_b, (halo, it_a, it_b)
+            assert same(chi_a, chi_b), halo
